@@ -1,0 +1,239 @@
+/* pct.h - C ABI of the MI355X (gfx950) multi-atlas patch-fusion library.
+ *
+ * Non-local multi-atlas patch fusion for pseudo-CT synthesis: for every
+ * interior target voxel v, every atlas a and every offset d of the NH^3
+ * search window,
+ *     SSD = sum_{p in [-1,1]^3} (T(v+p) - A_a(v+d+p))^2
+ *     w   = exp(-SSD / div),  div = 2*27*beta*sigma^2
+ *     out(v) = sum w*L'_a(v+d) / sum w
+ * with L' the atlas CT normalised to [0,255] (see pct_normalize_labels).
+ *
+ * Layout: float32 volumes, linear index z*nx*ny + x*ny + y (y fastest);
+ * atlas a starts at a*nx*ny*nz.  Identical to the reference program
+ * (src/CPU/NewPatchedBrainPureC.cpp:1288, 1300; src/GPU/PatchBasedPseudoCT.cu:169).
+ *
+ * Every entry point returns PCT_OK (0) or a negative pct_status; the library
+ * never calls exit().  Nothing here holds global problem state: dimensions
+ * travel in pct_problem instead of the reference's globals nx, ny, nz, num,
+ * nh (cpp:81-82, cu:652-654).
+ */
+#ifndef PCT_PCT_H
+#define PCT_PCT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCT_ABI_VERSION 1
+
+/* Same object as hipStream_t; declared here so that the header needs no HIP
+ * headers.  NULL means the default stream. */
+typedef struct ihipStream_t* pct_stream_t;
+
+typedef enum pct_status {
+    PCT_OK = 0,
+    PCT_ERR_INVALID = -1,     /* bad argument (null pointer, NH < 1, patch != 3, ...) */
+    PCT_ERR_NO_DEVICE = -2,   /* no HIP device, or the device is not gfx950 */
+    PCT_ERR_HIP = -3,         /* a HIP runtime call failed */
+    PCT_ERR_NO_MEMORY = -4,   /* host or device allocation failed */
+    PCT_ERR_IO = -5,          /* file could not be opened / written */
+    PCT_ERR_FORMAT = -6,      /* malformed MatrixIO / configuration file */
+    PCT_ERR_DIMS = -7,        /* dimensions of the input files do not match */
+    PCT_ERR_UNSUPPORTED = -8, /* no kernel configuration fits the request */
+    PCT_ERR_TOO_LARGE = -9    /* request exceeds a documented maximum */
+} pct_status;
+
+/* EXEC_CHOICE of the GPU program (src/GPU/PatchBasedPseudoCT.cu:71-78).
+ * GM and GM2 run the direct kernel (one lane per voxel, 27-term SSD per
+ * candidate); SM runs the LDS-tiled kernel with separable box-sum reuse.
+ * All three compute the same function.  GPU_TEST_TIME is a CLI mode. */
+typedef enum pct_exec_choice {
+    PCT_CHOICE_GM = 0,
+    PCT_CHOICE_GM2 = 1,
+    PCT_CHOICE_SM = 2,
+    PCT_CHOICE_GPU_TEST_TIME = 4
+} pct_exec_choice;
+
+typedef struct pct_problem {
+    int nx, ny, nz;   /* volume dimensions (MatrixIO rows, cols, depth) */
+    int n_atlas;      /* number of atlas pairs (reference global `num`) */
+    int nh;           /* search window edge NH; half window = (NH-1)/2 (cpp:1243) */
+    int patch;        /* patch edge; must be 3 (cpp:1240) */
+    double sigma;     /* cpp:1560 */
+    double beta;      /* cpp:1561 */
+} pct_problem;
+
+const char* pct_strerror(int status);
+int pct_abi_version(void);
+
+/* ---- pre-processing (src/CPU/NewPatchedBrainPureC.cpp:1583-1601) ---- */
+
+/* div = 2*27*beta*sigma^2 in float64 (cpp:1601, cu:2154). */
+double pct_weight_divisor(double sigma, double beta);
+
+/* In place, float32: L' = 255.0f*((L+1024.0f)/4095.0f) (cpp:1583-1593,
+ * cu:2009-2020). Host buffer. */
+void pct_normalize_labels(float* label, size_t n);
+
+/* Same operation on a device buffer, on `stream`. */
+int pct_normalize_labels_dev(float* d_label, size_t n, pct_stream_t stream);
+
+/* ---- the hot path ---- */
+
+/* Replaces MultiPatchBasedSegmentationGpu (src/GPU/PatchBasedPseudoCT.cu:1635)
+ * and its CPU twin MultiAtlasPatchBasedSegmentationOpenMP (cpp:1350).
+ * Host buffers; allocates device memory, copies in, runs, copies `out` back
+ * and frees, like the reference wrapper (cu:1687-1744).  `out` receives every
+ * voxel: interior = fused value (NaN where |P| or |S| <= DBL_EPSILON,
+ * cpp:1437-1443), border = 0 (cu:1693).  `choice` is a pct_exec_choice
+ * (GM, GM2 or SM). */
+int pct_fuse_host(const pct_problem* pb, const float* img, const float* atlas,
+                  const float* atlas_label_norm, float* out, int choice);
+
+/* Device-pointer variant: inputs already resident in HBM, enqueued on
+ * `stream`, asynchronous.  Uses a per-device workspace owned by the library
+ * (grown on demand; calls that overlap on one device must be ordered on one
+ * stream).  No host synchronisation unless the workspace has to grow. */
+int pct_fuse_dev(const pct_problem* pb, const float* d_img, const float* d_atlas,
+                 const float* d_atlas_label_norm, float* d_out, int choice,
+                 pct_stream_t stream);
+
+/* Bytes of device workspace pct_fuse_dev_ws needs for this problem/choice. */
+size_t pct_fuse_workspace_size(const pct_problem* pb, int choice);
+
+/* Same as pct_fuse_dev with a caller-provided workspace of at least
+ * pct_fuse_workspace_size() bytes (256-byte aligned). */
+int pct_fuse_dev_ws(const pct_problem* pb, const float* d_img, const float* d_atlas,
+                    const float* d_atlas_label_norm, float* d_out, int choice,
+                    void* d_workspace, size_t workspace_bytes, pct_stream_t stream);
+
+/* ---- kernel configuration / autotuner ---- */
+
+/* Description of the tiled (SM) kernel configuration chosen for a problem. */
+typedef struct pct_tile_config {
+    int id;            /* index into the library's configuration table */
+    int seg_width;     /* lanes per y segment (outputs per segment = seg_width-2) */
+    int bx, bz;        /* per-lane output block in x and z */
+    int waves;         /* wavefronts per workgroup */
+    int atlas_group;   /* atlases per work item */
+    int lds_bytes;     /* dynamic LDS per workgroup */
+    long long items;   /* work items (workgroups) launched */
+    double est_cost;   /* cost-model estimate (arbitrary units) */
+    double measured_ms;/* <0 if not measured */
+} pct_tile_config;
+
+/* Number of tiled-kernel configurations compiled into the library. */
+int pct_tile_config_count(void);
+
+/* Cost-model choice (no GPU work). */
+int pct_tile_config_choose(const pct_problem* pb, pct_tile_config* out);
+
+/* Measured choice: runs every admissible configuration `reps` times on the
+ * given device inputs, keeps the fastest, and caches the decision for this
+ * problem shape for the rest of the process (later pct_fuse_* calls with the
+ * same shape use it).  Synchronises `stream`. */
+int pct_autotune(const pct_problem* pb, const float* d_img, const float* d_atlas,
+                 const float* d_atlas_label_norm, float* d_out, int reps,
+                 pct_stream_t stream, pct_tile_config* best);
+
+/* Force a configuration (by id, atlas_group) for a problem shape; -1 clears. */
+int pct_tile_config_force(int id, int atlas_group);
+
+/* ---- kernel timing (no counterpart in the reference, which times the whole
+ * wrapper with cudaEvents, cu:2152-2188) ---- */
+
+/* While enabled, every fusion kernel launch (tiled or direct; not the
+ * finalize pass) is bracketed by HIP events on its own stream.  Enabling
+ * clears the record.  At most 4096 launches are recorded. */
+int pct_kernel_timing(int enable);
+
+/* Waits for the recorded events; returns the summed kernel time and the
+ * number of launches recorded since timing was enabled. */
+int pct_kernel_timing_read(double* total_ms, int* launches);
+
+/* ---- post-processing (cpp:1020-1106, 1221-1227, 1725-1739), host ---- */
+
+/* NaN fill with the median of the 3x3x3 neighbourhood, iterated in place in
+ * the reference's z, x, y order over the interior (cpp:1059-1106).  Returns
+ * the number of passes, or a negative status.  Where a NaN voxel has no
+ * non-NaN neighbour the reference reads uninitialised memory (cpp:1036-1041);
+ * here it stays NaN for that pass, and the loop stops if a pass fills
+ * nothing. */
+int pct_regularize(int nh, int nx, int ny, int nz, float* arr);
+
+/* round() every element (cpp:1221-1227). */
+void pct_round(float* arr, size_t n);
+
+/* v = round(4095.0f*(v/255.0f)) - 1024.0f (cpp:1733-1738). */
+void pct_to_hu(float* arr, size_t n);
+
+/* The whole chain of cpp:1725-1739: regularize, round, to HU. */
+int pct_postprocess(int nh, int nx, int ny, int nz, float* arr);
+
+/* ---- on-disk formats and inputs ---- */
+
+/* MatrixIO text format (cpp:252-320): lines num, rows, cols, depth, then the
+ * values in memory order separated by blanks.  Allocates *data (free with
+ * pct_free); dims4 = {rows, cols, depth, num} as the reference's
+ * `dimensions` vector (cpp:268-271). */
+int pct_matrixio_read(const char* path, float** data, int dims4[4]);
+
+/* Writes the MatrixIO text format with the reference's formatting (default
+ * 6 significant digits, one blank after every value; cpp:289-320). */
+int pct_matrixio_write(const char* path, const float* data, int num, int rows, int cols,
+                       int depth);
+
+/* Inputs as the reference's readMatFile (cpp:908-1018) fills its globals. */
+typedef struct pct_volume_set {
+    int nx, ny, nz, n_atlas;
+    float* img;     /* nx*ny*nz */
+    float* atlas;   /* n_atlas*nx*ny*nz */
+    float* label;   /* n_atlas*nx*ny*nz, CT in HU (not yet normalised) */
+} pct_volume_set;
+
+/* Configuration file (cpp:908-960): line 1 a random flag; if set, lines
+ * num, nx, ny, nz and uniform 10*U[0,1) data (cpp:961-973), drawn from a
+ * seeded generator (the reference seeds rand() from the clock, cpp:918);
+ * otherwise the atlas, label and image MatrixIO paths.  Buffers are
+ * allocated; release with pct_volume_set_free. */
+int pct_read_config(const char* path, uint64_t seed, pct_volume_set* out);
+void pct_volume_set_free(pct_volume_set* vs);
+void pct_free(void* p);
+
+/* Seeded uniform data of the reference's random mode (cpp:961-973):
+ * every value 10*U[0,1). */
+int pct_synth_uniform(int nx, int ny, int nz, int n_atlas, uint64_t seed,
+                      float* img, float* atlas, float* label);
+
+/* Seeded tissue-class phantom: N(0,1) noise, separable Gaussian blur
+ * (sigma 3 voxels, taps to 4 sigma, mirror boundary), thresholds at the
+ * 25/50/75 % quantiles -> classes {air, fat, soft tissue, bone};
+ * MRI = {0, 0.8, 0.5, 0.2}[class] + N(0, 0.03),
+ * CT  = {-1000, -100, 40, 1000}[class] + N(0, 20) HU.
+ * Atlas a: if independent_anatomy, its own class map; otherwise the
+ * target's class map shifted by a random integer vector in
+ * [-max_shift, max_shift]^3 (edge-clamped).  Every atlas has its own noise.
+ * atlas_ct is in HU (normalise with pct_normalize_labels). */
+int pct_synth_tissue(int nx, int ny, int nz, int n_atlas, int max_shift,
+                     int independent_anatomy, uint64_t seed,
+                     float* target_mri, float* atlas_mri, float* atlas_ct);
+
+/* ---- devices ---- */
+
+/* Number of HIP devices (0 without a GPU; never fails). */
+int pct_device_count(void);
+
+/* Makes `device` current for the calling thread (cudaSetDevice(argv[7]) of
+ * the reference, cu:1988); PCT_ERR_NO_DEVICE if it is not a gfx950 GPU. */
+int pct_set_device(int device);
+
+/* Releases the library's per-device workspaces and cached state. */
+void pct_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PCT_PCT_H */

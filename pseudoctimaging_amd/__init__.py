@@ -1,0 +1,19 @@
+"""MI355X-native (gfx950) multi-atlas patch fusion for pseudo-CT synthesis.
+
+The fusion runs in hand-written HIP kernels inside libpct.so (C ABI in
+include/pct/pct.h); this package is the Python host interface over it.
+"""
+from .api import (CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, autotune,  # noqa: F401
+                  device_count, force_tile_config, fuse_device, fused_to_hu_raw,
+                  matrixio_read, matrixio_write, multi_patch_based_segmentation_gpu,
+                  normalize_labels, normalize_labels_device, postprocess, problem, pseudo_ct,
+                  read_config, regularization_step, round_matrix, synth_tissue, synth_uniform,
+                  tile_config, to_hu, weight_divisor)
+
+__all__ = [
+    "CHOICE_GM", "CHOICE_GM2", "CHOICE_SM", "PctError", "autotune", "device_count",
+    "force_tile_config", "fuse_device", "fused_to_hu_raw", "matrixio_read", "matrixio_write",
+    "multi_patch_based_segmentation_gpu", "normalize_labels", "normalize_labels_device",
+    "postprocess", "problem", "pseudo_ct", "read_config", "regularization_step", "round_matrix",
+    "synth_tissue", "synth_uniform", "tile_config", "to_hu", "weight_divisor",
+]

@@ -1,0 +1,250 @@
+"""Python host interface over libpct.so, mirroring the reference program's
+operator surface for the fusion path.
+
+Arrays follow the reference layout: a volume is a C-contiguous float32 array
+of shape (nz, nx, ny) (linear index z*nx*ny + x*ny + y,
+src/CPU/NewPatchedBrainPureC.cpp:1300) and atlases stack on a leading axis,
+shape (n, nz, nx, ny) (cpp:1288).
+
+Reference entry point -> this module:
+  MultiPatchBasedSegmentationGpu (src/GPU/PatchBasedPseudoCT.cu:1635)
+      -> multi_patch_based_segmentation_gpu (host arrays) / fuse_device (HBM)
+  MultiAtlasPatchBasedSegmentationOpenMP (cpp:1350)
+      -> the same function; the fusion has no CPU implementation here
+  label normalisation (cpp:1583-1593)   -> normalize_labels
+  div = 2*27*beta*sigma^2 (cpp:1601)    -> weight_divisor
+  RegularizationStep / RoundMatrix / HU rescale (cpp:1059-1106, 1221-1227,
+  1725-1739)                            -> regularization_step, round_matrix,
+                                           to_hu, postprocess
+  MatrixIO<float>::Read / Store (cpp:252-320) -> matrixio_read / matrixio_write
+  readMatFile (cpp:908-1018)            -> read_config
+  main() (cpp:1502-1781, cu:1898-2239)  -> pseudo_ct (and the pct_cli program)
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_int
+
+import numpy as np
+
+from . import _lib
+from ._lib import (CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, Problem,  # noqa: F401
+                   TileConfig, check, fptr, lib)
+
+
+def _f32(a, name: str) -> np.ndarray:
+    a = np.asarray(a)
+    if a.dtype != np.float32:
+        raise TypeError(f"{name} must be float32, got {a.dtype}")
+    if not a.flags.c_contiguous:
+        raise ValueError(f"{name} must be C-contiguous")
+    return a
+
+
+def problem(shape, n_atlas: int, nh: int, sigma: float = 1.0, beta: float = 1.0) -> Problem:
+    """pct_problem for a volume of shape (nz, nx, ny)."""
+    nz, nx, ny = (int(s) for s in shape)
+    return Problem(nx=nx, ny=ny, nz=nz, n_atlas=int(n_atlas), nh=int(nh), patch=3,
+                   sigma=float(sigma), beta=float(beta))
+
+
+def weight_divisor(sigma: float, beta: float) -> float:
+    """div = 2*27*beta*sigma^2 (cpp:1601)."""
+    return lib().pct_weight_divisor(float(sigma), float(beta))
+
+
+def normalize_labels(label: np.ndarray) -> np.ndarray:
+    """In place: L' = 255*((L+1024)/4095) in float32 (cpp:1583-1593)."""
+    label = _f32(label, "label")
+    lib().pct_normalize_labels(fptr(label), label.size)
+    return label
+
+
+def device_count() -> int:
+    return lib().pct_device_count()
+
+
+def multi_patch_based_segmentation_gpu(img, atlas, atlas_label, nh: int, sigma: float = 1.0,
+                                       beta: float = 1.0, choice: int = CHOICE_SM) -> np.ndarray:
+    """Fused value of every voxel (interior: P/S or NaN, border: 0).
+
+    `atlas_label` is the normalised CT (see normalize_labels).  Host arrays in
+    and out; device memory is allocated and released per call like the
+    reference wrapper (cu:1687-1744).
+    """
+    img = _f32(img, "img")
+    atlas = _f32(atlas, "atlas")
+    atlas_label = _f32(atlas_label, "atlas_label")
+    if img.ndim != 3 or atlas.ndim != 4 or atlas.shape != atlas_label.shape \
+            or atlas.shape[1:] != img.shape:
+        raise ValueError("expected img (nz,nx,ny) and atlas/atlas_label (n,nz,nx,ny)")
+    pb = problem(img.shape, atlas.shape[0], nh, sigma, beta)
+    out = np.zeros(img.shape, np.float32)
+    check(lib().pct_fuse_host(byref(pb), fptr(img), fptr(atlas), fptr(atlas_label), fptr(out),
+                              int(choice)), "pct_fuse_host")
+    return out
+
+
+def _dev_ptr(t, name: str):
+    import torch
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise TypeError(f"{name} must be a device tensor")
+    if t.dtype != torch.float32 or not t.is_contiguous():
+        raise TypeError(f"{name} must be contiguous float32")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def fuse_device(img, atlas, atlas_label, out, nh: int, sigma: float = 1.0, beta: float = 1.0,
+                choice: int = CHOICE_SM, stream=None) -> None:
+    """Fusion on tensors already resident in HBM (torch only carries memory
+    and the stream).  Enqueued on `stream` (default: torch's current one)."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream(img.device)
+    pb = problem(img.shape, atlas.shape[0], nh, sigma, beta)
+    check(lib().pct_fuse_dev(byref(pb), _dev_ptr(img, "img"), _dev_ptr(atlas, "atlas"),
+                             _dev_ptr(atlas_label, "atlas_label"), _dev_ptr(out, "out"),
+                             int(choice), ctypes.c_void_p(stream.cuda_stream)), "pct_fuse_dev")
+
+
+def normalize_labels_device(label, stream=None) -> None:
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream(label.device)
+    check(lib().pct_normalize_labels_dev(_dev_ptr(label, "label"), label.numel(),
+                                         ctypes.c_void_p(stream.cuda_stream)),
+          "pct_normalize_labels_dev")
+
+
+def tile_config(shape, n_atlas: int, nh: int) -> dict:
+    """The tiled-kernel configuration the library would use (no GPU work)."""
+    pb = problem(shape, n_atlas, nh)
+    cfg = TileConfig()
+    check(lib().pct_tile_config_choose(byref(pb), byref(cfg)), "pct_tile_config_choose")
+    return cfg.as_dict()
+
+
+def autotune(img, atlas, atlas_label, out, nh: int, sigma: float = 1.0, beta: float = 1.0,
+             reps: int = 2, stream=None) -> dict:
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream(img.device)
+    pb = problem(img.shape, atlas.shape[0], nh, sigma, beta)
+    cfg = TileConfig()
+    check(lib().pct_autotune(byref(pb), _dev_ptr(img, "img"), _dev_ptr(atlas, "atlas"),
+                             _dev_ptr(atlas_label, "atlas_label"), _dev_ptr(out, "out"),
+                             int(reps), ctypes.c_void_p(stream.cuda_stream), byref(cfg)),
+          "pct_autotune")
+    return cfg.as_dict()
+
+
+def force_tile_config(config_id: int, atlas_group: int = 0) -> None:
+    check(lib().pct_tile_config_force(int(config_id), int(atlas_group)), "pct_tile_config_force")
+
+
+# ---- post-processing --------------------------------------------------------
+
+def regularization_step(arr: np.ndarray, nh: int) -> int:
+    """In-place NaN median fill (cpp:1059-1106); returns the number of passes."""
+    arr = _f32(arr, "arr")
+    nz, nx, ny = arr.shape
+    rc = lib().pct_regularize(int(nh), nx, ny, nz, fptr(arr))
+    if rc < 0:
+        check(rc, "pct_regularize")
+    return rc
+
+
+def round_matrix(arr: np.ndarray) -> np.ndarray:
+    arr = _f32(arr, "arr")
+    lib().pct_round(fptr(arr), arr.size)
+    return arr
+
+
+def to_hu(arr: np.ndarray) -> np.ndarray:
+    arr = _f32(arr, "arr")
+    lib().pct_to_hu(fptr(arr), arr.size)
+    return arr
+
+
+def postprocess(arr: np.ndarray, nh: int) -> np.ndarray:
+    """The chain of cpp:1725-1739 in place: NaN fill, round, back to HU."""
+    arr = _f32(arr, "arr")
+    nz, nx, ny = arr.shape
+    check(lib().pct_postprocess(int(nh), nx, ny, nz, fptr(arr)), "pct_postprocess")
+    return arr
+
+
+def pseudo_ct(img, atlas_mri, atlas_ct_hu, nh: int, sigma: float = 1.0, beta: float = 1.0,
+              choice: int = CHOICE_SM):
+    """main()'s pipeline (cpp:1583-1739): normalise the atlas CT, fuse,
+    post-process.  Returns (fused value before rounding, final HU volume)."""
+    label = np.array(atlas_ct_hu, dtype=np.float32, copy=True, order="C")
+    normalize_labels(label)
+    fused = multi_patch_based_segmentation_gpu(img, atlas_mri, label, nh, sigma, beta, choice)
+    hu = postprocess(fused.copy(), nh)
+    return fused, hu
+
+
+def fused_to_hu_raw(fused: np.ndarray) -> np.ndarray:
+    """HU of the fused value before any rounding: 4095*v/255 - 1024 (float64)."""
+    return 4095.0 * (np.asarray(fused, np.float64) / 255.0) - 1024.0
+
+
+# ---- files and inputs -------------------------------------------------------
+
+def matrixio_read(path: str):
+    """(array of shape (num, depth, rows, cols) squeezed to the reference's
+    memory order, dims) where dims = [rows, cols, depth, num] (cpp:252-287)."""
+    data = _lib.FP()
+    dims = (c_int * 4)()
+    check(lib().pct_matrixio_read(path.encode(), byref(data), dims), f"read {path}")
+    rows, cols, depth, num = list(dims)
+    n = rows * cols * depth * num
+    arr = np.ctypeslib.as_array(data, shape=(n,)).copy()
+    lib().pct_free(ctypes.cast(data, ctypes.c_void_p))
+    return arr, [rows, cols, depth, num]
+
+
+def matrixio_write(path: str, arr, num: int, rows: int, cols: int, depth: int) -> None:
+    arr = _f32(np.ascontiguousarray(arr), "arr")
+    if arr.size != num * rows * cols * depth:
+        raise ValueError("size does not match dimensions")
+    check(lib().pct_matrixio_write(path.encode(), fptr(arr), num, rows, cols, depth),
+          f"write {path}")
+
+
+def read_config(path: str, seed: int = 0) -> dict:
+    """readMatFile (cpp:908-1018): returns img (nz,nx,ny), atlas and label
+    (n,nz,nx,ny) with the label still in HU."""
+    vs = _lib.VolumeSet()
+    check(lib().pct_read_config(path.encode(), int(seed), byref(vs)), f"read_config {path}")
+    try:
+        nz, nx, ny, n = vs.nz, vs.nx, vs.ny, vs.n_atlas
+        N = nx * ny * nz
+        img = np.ctypeslib.as_array(vs.img, shape=(N,)).copy().reshape(nz, nx, ny)
+        atlas = np.ctypeslib.as_array(vs.atlas, shape=(N * n,)).copy().reshape(n, nz, nx, ny)
+        label = np.ctypeslib.as_array(vs.label, shape=(N * n,)).copy().reshape(n, nz, nx, ny)
+    finally:
+        lib().pct_volume_set_free(byref(vs))
+    return {"img": img, "atlas": atlas, "label": label}
+
+
+def synth_tissue(nx: int, ny: int, nz: int, n_atlas: int, max_shift: int,
+                 independent_anatomy: bool = False, seed: int = 0):
+    """Seeded tissue-class phantom: (target MRI, atlas MRI, atlas CT in HU)."""
+    img = np.empty((nz, nx, ny), np.float32)
+    amri = np.empty((n_atlas, nz, nx, ny), np.float32)
+    act = np.empty((n_atlas, nz, nx, ny), np.float32)
+    check(lib().pct_synth_tissue(nx, ny, nz, n_atlas, max_shift, int(bool(independent_anatomy)),
+                                 int(seed), fptr(img), fptr(amri), fptr(act)), "synth_tissue")
+    return img, amri, act
+
+
+def synth_uniform(nx: int, ny: int, nz: int, n_atlas: int, seed: int = 0):
+    """Seeded data of the reference's random mode: every value 10*U[0,1)."""
+    img = np.empty((nz, nx, ny), np.float32)
+    atlas = np.empty((n_atlas, nz, nx, ny), np.float32)
+    label = np.empty((n_atlas, nz, nx, ny), np.float32)
+    check(lib().pct_synth_uniform(nx, ny, nz, n_atlas, int(seed), fptr(img), fptr(atlas),
+                                  fptr(label)), "synth_uniform")
+    return img, atlas, label

@@ -1,0 +1,980 @@
+// Plain FP32 multi-atlas patch fusion for gfx950 (MI355X).
+//
+// Semantics follow the reference fusion loop (src/CPU/NewPatchedBrainPureC.cpp:
+// 1184-1212 PatchSimilarity, 1350-1449 the OpenMP loop nest; the CUDA kernels
+// src/GPU/PatchBasedPseudoCT.cu:162-641 compute the same function).  The
+// design is new:
+//
+//  * Tiled kernel (EXEC_CHOICE SM).  For a fixed atlas a and offset d the
+//    27-term SSD of every target voxel is a 3x3x3 box sum of the squared
+//    difference D_d(u) = (T(u) - A_a(u+d))^2, so it is computed separably:
+//    x by register pairs, z by register pairs, y across lanes with two DPP
+//    adds (wave_shr:1 / wave_shl:1).  Lanes map to consecutive y (coalesced
+//    global and conflict-free LDS accesses); each lane owns a BZ x BX block
+//    of outputs in (z, x) and keeps the scaled target patch of that block in
+//    VGPRs.  Per (atlas, dz) the workgroup stages the atlas MRI planes it
+//    needs (BZ+2 planes) and the CT planes (BZ planes) for its tile in LDS;
+//    the (dx, dy) loop then reads LDS at compile-time offsets.  MRI values are
+//    pre-scaled by sqrt(log2(e)/div) so the weight is exp2(-SSD') with no
+//    multiply.  A work item is (tile, group of atlases); it writes per-group
+//    float partial sums P, S, and a finalize kernel adds the groups in a fixed
+//    order and applies the reference's NaN rule (cpp:1437-1443), so results
+//    are deterministic.
+//  * Direct kernel (EXEC_CHOICE GM / GM2).  One lane per voxel, the 27-term
+//    SSD per candidate straight from global memory, FP32.  An independent
+//    GPU implementation used as the GM/GM2 surface and as a cross-check.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "pct_internal.h"
+
+namespace pct {
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+
+// lane l receives lane l-1 (lane 0 receives 0).
+__device__ __forceinline__ float lane_prev(float v) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+}
+// lane l receives lane l+1 (lane 63 receives 0).
+__device__ __forceinline__ float lane_next(float v) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+}
+
+// Work items are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous run of items so that tiles that share halos share an L2.
+__device__ __forceinline__ int xcd_contiguous(int b, int n) {
+    const int per = n >> 3, rem = n & 7;
+    const int xcd = b & 7, local = b >> 3;
+    return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+// The reference's rule (cpp:1437-1443) in float: DBL_EPSILON = 2^-52 is exact.
+__device__ __forceinline__ float fused_value(float P, float S) {
+    const float eps = 2.220446049250313080847e-16f;
+    return (fabsf(P) > eps && fabsf(S) > eps) ? (P / S) : __builtin_nanf("");
+}
+
+// ---------------------------------------------------------------------------
+// tiled kernel
+// ---------------------------------------------------------------------------
+
+struct TiledArgs {
+    const float* img;
+    const float* atlas;
+    const float* label;
+    float* ws;        // [n_groups][2][N]
+    long long N;      // nx*ny*nz
+    int nx, ny, nz;
+    int r;            // hw + 1
+    int tiles_y, tiles_x;
+    int n_tiles;
+    int n_items;
+    int n_atlas, group;
+    float scale;      // sqrt(log2(e)/div)
+};
+
+template <int W, int BX, int BZ, int NW, int HW>
+struct TiledCfg {
+    static constexpr int S = 64 / W;           // segments per wave
+    static constexpr int NSEG = NW * S;        // segments per workgroup (stacked in x)
+    static constexpr int WX = NSEG * BX;       // output rows per workgroup
+    static constexpr int OY = W - 2;           // outputs per segment in y
+    static constexpr int CW = W + 2 * HW;      // LDS row length (y)
+    static constexpr int RX = WX + 2 + 2 * HW; // LDS rows per plane (x)
+    static constexpr int PS = RX * CW;         // LDS plane stride
+    static constexpr int APLANES = BZ + 2;     // atlas MRI planes per stage
+    static constexpr int LPLANES = BZ;         // atlas CT planes per stage
+    static constexpr int LOFF = APLANES * PS;  // CT region offset
+    static constexpr int LDS_FLOATS = (APLANES + LPLANES) * PS;
+    static constexpr int LDS_BYTES = LDS_FLOATS * 4;
+    static constexpr int COLCHUNKS = (CW + 63) / 64;
+    // ds_read_b32 takes a 16-bit byte offset: planes are addressed from
+    // several base registers, PPB planes (plus row offsets) per base.
+    static constexpr int PPB = (16383 - (BX + 1) * CW) / PS + 1;
+    static_assert(PPB >= 1, "one plane must fit an LDS immediate window");
+    static_assert(64 % W == 0, "segment width must divide the wavefront");
+    static_assert(BX % 2 == 0 && BZ % 2 == 0, "pairwise box sums need even blocks");
+};
+
+// Hides an LDS element index from the optimiser so that it stays one base
+// register instead of being folded into offsets that do not fit an LDS
+// immediate.
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// One (dx, dy) candidate step for the lane's BZ x BX block.  `base` is the
+// LDS element index of plane 0, row 0 of the lane's window for this step.
+template <class C, int BX, int BZ>
+__device__ __forceinline__ void tiled_step(const float* __restrict__ lds, int base,
+                                           const float (&tv)[BZ + 2][BX + 2],
+                                           float (&P)[BZ][BX], float (&S)[BZ][BX]) {
+    constexpr int NBA = (BZ + 2 + C::PPB - 1) / C::PPB;   // bases for the MRI planes
+    constexpr int NBL = (BZ + C::PPB - 1) / C::PPB;       // bases for the CT planes
+    int ba[NBA];
+    int bl[NBL];
+#pragma unroll
+    for (int b = 0; b < NBA; ++b) ba[b] = b ? opaque(base + b * C::PPB * C::PS) : base;
+#pragma unroll
+    for (int b = 0; b < NBL; ++b) bl[b] = opaque(base + C::LOFF + b * C::PPB * C::PS);
+
+    float ex[BZ + 2][BX];
+#pragma unroll
+    for (int pz = 0; pz < BZ + 2; ++pz) {
+        float d[BX + 2];
+        const int pp = ba[pz / C::PPB] + (pz % C::PPB) * C::PS;
+#pragma unroll
+        for (int i = 0; i < BX + 2; ++i) {
+            const float df = tv[pz][i] - lds[pp + i * C::CW];
+            d[i] = df * df;
+        }
+#pragma unroll
+        for (int i = 0; i < BX; i += 2) {
+            const float q = d[i + 1] + d[i + 2];
+            ex[pz][i] = d[i] + q;
+            ex[pz][i + 1] = q + d[i + 3];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < BZ; j += 2) {
+#pragma unroll
+        for (int i = 0; i < BX; ++i) {
+            const float q = ex[j + 1][i] + ex[j + 2][i];
+            const float e0 = ex[j][i] + q;
+            const float e1 = q + ex[j + 3][i];
+            const float s0 = lane_next(e0) + (lane_prev(e0) + e0);
+            const float s1 = lane_next(e1) + (lane_prev(e1) + e1);
+            const float w0 = __builtin_amdgcn_exp2f(-s0);
+            const float w1 = __builtin_amdgcn_exp2f(-s1);
+            const float l0 = lds[bl[j / C::PPB] + (j % C::PPB) * C::PS + (i + 1) * C::CW];
+            const float l1 =
+                lds[bl[(j + 1) / C::PPB] + ((j + 1) % C::PPB) * C::PS + (i + 1) * C::CW];
+            P[j][i] = fmaf(w0, l0, P[j][i]);
+            S[j][i] += w0;
+            P[j + 1][i] = fmaf(w1, l1, P[j + 1][i]);
+            S[j + 1][i] += w1;
+        }
+    }
+}
+
+template <int W, int BX, int BZ, int NW, int HW>
+__global__ __launch_bounds__(NW * 64) void fuse_tiled(TiledArgs args) {
+    using C = TiledCfg<W, BX, BZ, NW, HW>;
+    extern __shared__ float lds[];
+
+    const int item = xcd_contiguous(blockIdx.x, args.n_items);
+    const int g = item / args.n_tiles;
+    int t = item - g * args.n_tiles;
+    const int ty = t % args.tiles_y;
+    t /= args.tiles_y;
+    const int tx = t % args.tiles_x;
+    const int tz = t / args.tiles_x;
+
+    const int tid = threadIdx.x;
+    const int wv = tid >> 6;
+    const int lane = tid & 63;
+    const int seg = lane / W;
+    const int ly = lane % W;
+    const int sg = wv * C::S + seg;
+
+    const int nx = args.nx, ny = args.ny, nz = args.nz, r = args.r;
+    const long long plane = (long long)nx * ny;
+    const int y0 = r + ty * C::OY;
+    const int x0 = r + tx * C::WX;
+    const int z0 = r + tz * BZ;
+    const int y = y0 - 1 + ly;        // >= 0 because r >= 1
+    const int xb = x0 + sg * BX;      // first output row of this lane
+
+    // Scaled target values of the lane's block plus a one-voxel halo in z
+    // and x (y halo comes from the neighbouring lanes).  Zero outside the
+    // volume: such values only reach outputs that are never stored.
+    float tv[BZ + 2][BX + 2];
+#pragma unroll
+    for (int pz = 0; pz < BZ + 2; ++pz)
+#pragma unroll
+        for (int i = 0; i < BX + 2; ++i) {
+            const int zz = z0 - 1 + pz, xx = xb - 1 + i;
+            const bool ok = zz < nz && xx < nx && y < ny;
+            tv[pz][i] = ok ? args.img[zz * plane + (long long)xx * ny + y] * args.scale : 0.f;
+        }
+
+    float P[BZ][BX], S[BZ][BX];
+#pragma unroll
+    for (int j = 0; j < BZ; ++j)
+#pragma unroll
+        for (int i = 0; i < BX; ++i) P[j][i] = S[j][i] = 0.f;
+
+    const int lane_base = sg * BX * C::CW + ly;
+    const int xs = x0 - 1 - HW;   // volume x of LDS row 0
+    const int ys = y0 - 1 - HW;   // volume y of LDS column 0
+    const int a_begin = g * args.group;
+    const int a_end = min(args.n_atlas, a_begin + args.group);
+
+    for (int a = a_begin; a < a_end; ++a) {
+        const float* __restrict__ A = args.atlas + (long long)a * args.N;
+        const float* __restrict__ L = args.label + (long long)a * args.N;
+        for (int dz = -HW; dz <= HW; ++dz) {
+            __syncthreads();
+            // Stage: LDS plane p < APLANES holds atlas MRI at z = z0+dz-1+p
+            // (scaled), plane APLANES+j holds atlas CT at z = z0+dz+j.
+#pragma unroll 1
+            for (int p = 0; p < C::APLANES + C::LPLANES; ++p) {
+                const bool is_mri = p < C::APLANES;
+                const int zz = is_mri ? z0 + dz - 1 + p : z0 + dz + (p - C::APLANES);
+                const float* __restrict__ src = is_mri ? A : L;
+                const float mul = is_mri ? args.scale : 1.f;
+                const bool zok = zz >= 0 && zz < nz;
+                const long long zoff = (long long)zz * plane;
+#pragma unroll 4
+                for (int rr = wv; rr < C::RX; rr += NW) {
+                    const int xx = xs + rr;
+                    const bool xok = zok && xx >= 0 && xx < nx;
+#pragma unroll
+                    for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
+                        const int c = lane + cc * 64;
+                        const int yy = ys + c;
+                        if (c < C::CW) {
+                            float v = 0.f;
+                            if (xok && yy >= 0 && yy < ny) v = src[zoff + (long long)xx * ny + yy] * mul;
+                            lds[p * C::PS + rr * C::CW + c] = v;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll 1
+            for (int dx = 0; dx <= 2 * HW; ++dx) {
+#pragma unroll 1
+                for (int dy = 0; dy <= 2 * HW; ++dy) {
+                    tiled_step<C, BX, BZ>(lds, lane_base + dx * C::CW + dy, tv, P, S);
+                }
+            }
+        }
+    }
+
+    // Store the group's partial sums for the lane's valid outputs.
+    if (ly >= 1 && ly <= W - 2 && y < ny - r) {
+        float* __restrict__ wsP = args.ws + (long long)(2 * g) * args.N;
+        float* __restrict__ wsS = wsP + args.N;
+#pragma unroll
+        for (int j = 0; j < BZ; ++j)
+#pragma unroll
+            for (int i = 0; i < BX; ++i) {
+                const int z = z0 + j, x = xb + i;
+                if (z < nz - r && x < nx - r) {
+                    const long long idx = z * plane + (long long)x * ny + y;
+                    wsP[idx] = P[j][i];
+                    wsS[idx] = S[j][i];
+                }
+            }
+    }
+}
+
+// Sum of the per-group partials in group order; border voxels get 0
+// (the reference zeroes its output, cu:1693 / cpp:1606).
+__global__ void fuse_finalize(const float* __restrict__ ws, int n_groups, long long N, int nx,
+                              int ny, int nz, int r, float* __restrict__ out) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    const int x = blockIdx.y;
+    const int z = blockIdx.z;
+    if (y >= ny) return;
+    const long long idx = ((long long)z * nx + x) * ny + y;
+    const bool interior = z >= r && z < nz - r && x >= r && x < nx - r && y >= r && y < ny - r;
+    float v = 0.f;
+    if (interior) {
+        float P = 0.f, S = 0.f;
+        for (int g = 0; g < n_groups; ++g) {
+            P += ws[(long long)(2 * g) * N + idx];
+            S += ws[(long long)(2 * g + 1) * N + idx];
+        }
+        v = fused_value(P, S);
+    }
+    out[idx] = v;
+}
+
+// ---------------------------------------------------------------------------
+// direct kernel (GM / GM2 surface)
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void fuse_direct(const float* __restrict__ img,
+                                                   const float* __restrict__ atlas,
+                                                   const float* __restrict__ label, long long N,
+                                                   int nx, int ny, int nz, int hw, int n_atlas,
+                                                   float k, float* __restrict__ out) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    const int x = blockIdx.y;
+    const int z = blockIdx.z;
+    if (y >= ny) return;
+    const int r = hw + 1;
+    const long long plane = (long long)nx * ny;
+    const long long idx = z * plane + (long long)x * ny + y;
+    if (!(z >= r && z < nz - r && x >= r && x < nx - r && y >= r && y < ny - r)) {
+        out[idx] = 0.f;
+        return;
+    }
+    float t[27];
+#pragma unroll
+    for (int p = 0; p < 27; ++p)
+        t[p] = img[idx + (p / 9 - 1) * plane + (long long)((p / 3) % 3 - 1) * ny + (p % 3 - 1)];
+    float P = 0.f, S = 0.f;
+    for (int a = 0; a < n_atlas; ++a) {
+        const float* __restrict__ A = atlas + (long long)a * N;
+        const float* __restrict__ L = label + (long long)a * N;
+        for (int dz = -hw; dz <= hw; ++dz)
+            for (int dx = -hw; dx <= hw; ++dx)
+                for (int dy = -hw; dy <= hw; ++dy) {
+                    const long long c = idx + dz * plane + (long long)dx * ny + dy;
+                    float ssd = 0.f;
+#pragma unroll
+                    for (int p = 0; p < 27; ++p) {
+                        const float df =
+                            t[p] - A[c + (p / 9 - 1) * plane + (long long)((p / 3) % 3 - 1) * ny +
+                                     (p % 3 - 1)];
+                        ssd = fmaf(df, df, ssd);
+                    }
+                    const float w = __builtin_amdgcn_exp2f(-ssd * k);
+                    P = fmaf(w, L[c], P);
+                    S += w;
+                }
+    }
+    out[idx] = fused_value(P, S);
+}
+
+// cpp:1583-1593, float32, same operation order.
+__global__ void normalize_labels_kernel(float* __restrict__ v, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        v[i] = 255.0f * ((v[i] + 1024.0f) / 4095.0f);
+}
+
+// ---------------------------------------------------------------------------
+// host side: configuration table, geometry, launches
+// ---------------------------------------------------------------------------
+
+using TiledFn = void (*)(TiledArgs);
+
+struct TiledEntry {
+    TileShape shape;
+    int hw;
+    TiledFn fn;
+    int lds_bytes;
+};
+
+template <int W, int BX, int BZ, int NW, int HW>
+TiledEntry make_entry() {
+    using C = TiledCfg<W, BX, BZ, NW, HW>;
+    return TiledEntry{{W, BX, BZ, NW}, HW, &fuse_tiled<W, BX, BZ, NW, HW>, C::LDS_BYTES};
+}
+
+template <int W, int BX, int BZ, int NW>
+void add_shape(std::vector<TiledEntry>& v) {
+    v.push_back(make_entry<W, BX, BZ, NW, 1>());
+    v.push_back(make_entry<W, BX, BZ, NW, 2>());
+    v.push_back(make_entry<W, BX, BZ, NW, 3>());
+    v.push_back(make_entry<W, BX, BZ, NW, 4>());
+    v.push_back(make_entry<W, BX, BZ, NW, 5>());
+    v.push_back(make_entry<W, BX, BZ, NW, 6>());
+    v.push_back(make_entry<W, BX, BZ, NW, 7>());
+}
+
+// The shapes compiled into the library (index = pct_tile_config.id).
+static const std::vector<TileShape>& shapes() {
+    static const std::vector<TileShape> s = {
+        {64, 4, 8, 4},   // 0: one 62-wide y segment per wave, 16 x rows per workgroup
+        {32, 4, 8, 4},   // 1: two 30-wide y segments per wave, 32 x rows
+        {32, 4, 4, 2},   // 2: small LDS footprint, several workgroups per CU
+        {32, 2, 8, 4},   // 3: 16 x rows, 8 planes
+        {64, 4, 4, 4},   // 4: 16 x rows, 4 planes
+    };
+    return s;
+}
+
+static const std::vector<TiledEntry>& entries() {
+    static const std::vector<TiledEntry> e = [] {
+        std::vector<TiledEntry> v;
+        add_shape<64, 4, 8, 4>(v);
+        add_shape<32, 4, 8, 4>(v);
+        add_shape<32, 4, 4, 2>(v);
+        add_shape<32, 2, 8, 4>(v);
+        add_shape<64, 4, 4, 4>(v);
+        return v;
+    }();
+    return e;
+}
+
+static const TiledEntry* find_entry(int shape_id, int hw) {
+    if (shape_id < 0 || shape_id >= (int)shapes().size()) return nullptr;
+    if (hw < 1 || hw > kMaxTiledHalfWindow) return nullptr;
+    const auto& e = entries()[shape_id * kMaxTiledHalfWindow + (hw - 1)];
+    return &e;
+}
+
+constexpr int kLdsPerCu = 160 * 1024;
+constexpr int kNumCu = 256;
+
+int validate_problem(const pct_problem* pb) {
+    if (!pb) return PCT_ERR_INVALID;
+    if (pb->nx < 1 || pb->ny < 1 || pb->nz < 1 || pb->n_atlas < 1) return PCT_ERR_INVALID;
+    if (pb->nh < 1 || pb->patch != 3) return PCT_ERR_INVALID;
+    if (!(pb->sigma > 0.0) || !(pb->beta > 0.0)) return PCT_ERR_INVALID;
+    if (!std::isfinite(pb->sigma) || !std::isfinite(pb->beta)) return PCT_ERR_INVALID;
+    const long long N = (long long)pb->nx * pb->ny * pb->nz;
+    if (pb->nx > 65535 || pb->ny > 65535 || pb->nz > 65535) return PCT_ERR_TOO_LARGE;
+    if (N * (long long)pb->n_atlas > (1LL << 40)) return PCT_ERR_TOO_LARGE;
+    return PCT_OK;
+}
+
+static int interior(int n, int r) { return std::max(0, n - 2 * r); }
+
+static bool make_geometry(const pct_problem* pb, int shape_id, int group, TileGeometry* g) {
+    const int hw = (pb->nh - 1) / 2;
+    const TiledEntry* e = find_entry(shape_id, hw);
+    if (!e) return false;
+    const TileShape& s = e->shape;
+    if (e->lds_bytes > kLdsPerCu) return false;
+    const int S = 64 / s.seg_width;
+    const int WX = s.waves * S * s.bx;
+    const int OY = s.seg_width - 2;
+    g->hw = hw;
+    g->r = hw + 1;
+    const int iy = interior(pb->ny, g->r), ix = interior(pb->nx, g->r), iz = interior(pb->nz, g->r);
+    g->tiles_y = (iy + OY - 1) / OY;
+    g->tiles_x = (ix + WX - 1) / WX;
+    g->tiles_z = (iz + s.bz - 1) / s.bz;
+    g->n_tiles = (long long)g->tiles_y * g->tiles_x * g->tiles_z;
+    g->group = std::max(1, std::min(group, pb->n_atlas));
+    g->n_groups = (pb->n_atlas + g->group - 1) / g->group;
+    g->n_items = g->n_tiles * g->n_groups;
+    g->lds_bytes = e->lds_bytes;
+    const int by_lds = kLdsPerCu / e->lds_bytes;
+    const int by_waves = 32 / s.waves;  // 8 waves per SIMD slot budget, 4 SIMDs
+    g->wg_per_cu = std::max(1, std::min(by_lds, by_waves));
+    return g->n_items < INT_MAX;
+}
+
+// Cost model: issue slots per output-candidate for one lane, including the
+// halo lanes and halo rows, times the number of rounds the grid needs.
+static double model_cost(const pct_problem* pb, int shape_id, int group, TileGeometry* geo) {
+    TileGeometry g;
+    if (!make_geometry(pb, shape_id, group, &g)) return -1;
+    const TileShape& s = shapes()[shape_id];
+    const double bx = s.bx, bz = s.bz;
+    const double nh = 2 * g.hw + 1;
+    // VALU slots per (dx, dy) step per lane (exp counted as 4 slots)
+    const double step = (bz + 2) * (bx + 2) * 2 + (bz + 2) * bx * 1.5 + bz * bx * (1.5 + 2 + 4 + 2);
+    // staging slots per dz step per lane (loads + stores + scaling)
+    const int S = 64 / s.seg_width;
+    const double rows = (2 * bz + 2) * (s.waves * S * s.bx + 2 + 2 * g.hw);
+    const double stage = rows / s.waves * 6.0 * ((s.seg_width + 2 * g.hw + 63) / 64);
+    const double wave_time = g.group * nh * (nh * nh * step + stage);
+    // a CU runs 4 SIMDs; waves of all resident workgroups share them
+    const double waves_per_cu = g.wg_per_cu * s.waves;
+    const double cu_rate = std::min(4.0, waves_per_cu) / waves_per_cu;  // share of a SIMD per wave
+    const double item_time = wave_time / cu_rate / g.wg_per_cu;          // per item, amortised
+    const double slots = (double)kNumCu * g.wg_per_cu;
+    const double rounds = std::ceil((double)g.n_items / slots);
+    if (geo) *geo = g;
+    return rounds * item_time * g.wg_per_cu;
+}
+
+struct ShapeKey {
+    int nx, ny, nz, n, nh;
+    bool operator<(const ShapeKey& o) const {
+        return std::tie(nx, ny, nz, n, nh) < std::tie(o.nx, o.ny, o.nz, o.n, o.nh);
+    }
+};
+
+static std::mutex g_cfg_mutex;
+static std::map<ShapeKey, std::pair<int, int>> g_tuned;   // shape -> (shape_id, group)
+static int g_forced_id = -1, g_forced_group = 0;
+
+static bool choose(const pct_problem* pb, int* shape_id, int* group, double* cost,
+                   TileGeometry* geo) {
+    {
+        std::lock_guard<std::mutex> lk(g_cfg_mutex);
+        if (g_forced_id >= 0) {
+            *shape_id = g_forced_id;
+            *group = g_forced_group > 0 ? g_forced_group : 1;
+            double c = model_cost(pb, *shape_id, *group, geo);
+            if (cost) *cost = c;
+            return c >= 0;
+        }
+        auto it = g_tuned.find(ShapeKey{pb->nx, pb->ny, pb->nz, pb->n_atlas, pb->nh});
+        if (it != g_tuned.end()) {
+            *shape_id = it->second.first;
+            *group = it->second.second;
+            double c = model_cost(pb, *shape_id, *group, geo);
+            if (cost) *cost = c;
+            return c >= 0;
+        }
+    }
+    double best = -1;
+    for (int sid = 0; sid < (int)shapes().size(); ++sid)
+        for (int grp = 1; grp <= pb->n_atlas; ++grp) {
+            TileGeometry g;
+            const double c = model_cost(pb, sid, grp, &g);
+            if (c < 0) continue;
+            if (best < 0 || c < best * 0.999) {
+                best = c;
+                *shape_id = sid;
+                *group = grp;
+                if (geo) *geo = g;
+            }
+        }
+    if (cost) *cost = best;
+    return best >= 0;
+}
+
+// ---------------------------------------------------------------------------
+// device / workspace management
+// ---------------------------------------------------------------------------
+
+struct DeviceState {
+    bool checked = false;
+    bool ok = false;
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+};
+
+static std::mutex g_dev_mutex;
+static std::map<int, DeviceState> g_dev;
+
+static int check_device(int* dev_out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return PCT_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> lk(g_dev_mutex);
+    DeviceState& st = g_dev[dev];
+    if (!st.checked) {
+        hipDeviceProp_t prop;
+        st.checked = true;
+        st.ok = hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+                std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+        if (st.ok) {
+            for (const auto& e : entries())
+                if (e.lds_bytes <= kLdsPerCu)
+                    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(e.fn),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              e.lds_bytes);
+        }
+    }
+    if (dev_out) *dev_out = dev;
+    return st.ok ? PCT_OK : PCT_ERR_NO_DEVICE;
+}
+
+static int ensure_workspace(int dev, size_t bytes, void** ptr) {
+    std::lock_guard<std::mutex> lk(g_dev_mutex);
+    DeviceState& st = g_dev[dev];
+    if (st.ws_bytes < bytes) {
+        if (st.ws) {
+            if (hipDeviceSynchronize() != hipSuccess) return PCT_ERR_HIP;
+            (void)hipFree(st.ws);
+            st.ws = nullptr;
+            st.ws_bytes = 0;
+        }
+        if (hipMalloc(&st.ws, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return PCT_ERR_NO_MEMORY;
+        }
+        st.ws_bytes = bytes;
+    }
+    *ptr = st.ws;
+    return PCT_OK;
+}
+
+static size_t tiled_ws_bytes(const pct_problem* pb, int n_groups) {
+    const size_t N = (size_t)pb->nx * pb->ny * pb->nz;
+    return ((2 * (size_t)n_groups * N * sizeof(float)) + 255) & ~(size_t)255;
+}
+
+static bool is_tiled_choice(const pct_problem* pb, int choice) {
+    return choice == PCT_CHOICE_SM && (pb->nh - 1) / 2 <= kMaxTiledHalfWindow &&
+           (pb->nh - 1) / 2 >= 1;
+}
+
+// Optional per-launch HIP events around the fusion kernels (see
+// pct_kernel_timing).  Events are created once and reused.
+struct KernelTimer {
+    std::mutex mu;
+    bool enabled = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+    size_t used = 0;
+    static constexpr size_t kMax = 4096;
+
+    // Returns the event pair to record around the next launch, or nulls.
+    std::pair<hipEvent_t, hipEvent_t> next() {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!enabled || used >= kMax) return {nullptr, nullptr};
+        if (used == pool.size()) {
+            hipEvent_t a = nullptr, b = nullptr;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+                return {nullptr, nullptr};
+            pool.emplace_back(a, b);
+        }
+        return pool[used++];
+    }
+};
+static KernelTimer g_timer;
+
+struct TimedLaunch {
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    hipStream_t s;
+    explicit TimedLaunch(hipStream_t st) : ev(g_timer.next()), s(st) {
+        if (ev.first) (void)hipEventRecord(ev.first, s);
+    }
+    ~TimedLaunch() {
+        if (ev.second) (void)hipEventRecord(ev.second, s);
+    }
+};
+
+static int launch_tiled(const pct_problem* pb, const float* img, const float* atlas,
+                        const float* label, float* out, void* ws, size_t ws_bytes,
+                        int shape_id, const TileGeometry& g, hipStream_t stream) {
+    const TiledEntry* e = find_entry(shape_id, g.hw);
+    if (!e) return PCT_ERR_UNSUPPORTED;
+    if (ws_bytes < tiled_ws_bytes(pb, g.n_groups)) return PCT_ERR_INVALID;
+    const long long N = (long long)pb->nx * pb->ny * pb->nz;
+    TiledArgs a;
+    a.img = img;
+    a.atlas = atlas;
+    a.label = label;
+    a.ws = static_cast<float*>(ws);
+    a.N = N;
+    a.nx = pb->nx;
+    a.ny = pb->ny;
+    a.nz = pb->nz;
+    a.r = g.r;
+    a.tiles_y = g.tiles_y;
+    a.tiles_x = g.tiles_x;
+    a.n_tiles = (int)g.n_tiles;
+    a.n_items = (int)g.n_items;
+    a.n_atlas = pb->n_atlas;
+    a.group = g.group;
+    const double div = pct_weight_divisor(pb->sigma, pb->beta);
+    a.scale = (float)std::sqrt(1.4426950408889634 / div);
+    if (g.n_items > 0) {
+        void* kargs[] = {&a};
+        TimedLaunch timed(stream);
+        if (hipLaunchKernel(reinterpret_cast<const void*>(e->fn), dim3((unsigned)g.n_items),
+                            dim3(e->shape.waves * 64), kargs, (size_t)e->lds_bytes,
+                            stream) != hipSuccess)
+            return PCT_ERR_HIP;
+    }
+    const dim3 fb(256), fgrid((pb->ny + 255) / 256, pb->nx, pb->nz);
+    hipLaunchKernelGGL(fuse_finalize, fgrid, fb, 0, stream, static_cast<const float*>(ws),
+                       g.n_items > 0 ? g.n_groups : 0, N, pb->nx, pb->ny, pb->nz, g.r, out);
+    if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
+    return PCT_OK;
+}
+
+static int launch_direct(const pct_problem* pb, const float* img, const float* atlas,
+                         const float* label, float* out, hipStream_t stream) {
+    const long long N = (long long)pb->nx * pb->ny * pb->nz;
+    const int hw = (pb->nh - 1) / 2;
+    const double div = pct_weight_divisor(pb->sigma, pb->beta);
+    const float k = (float)(1.4426950408889634 / div);
+    const dim3 block(256), grid((pb->ny + 255) / 256, pb->nx, pb->nz);
+    TimedLaunch timed(stream);
+    hipLaunchKernelGGL(fuse_direct, grid, block, 0, stream, img, atlas, label, N, pb->nx, pb->ny,
+                       pb->nz, hw, pb->n_atlas, k, out);
+    return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;
+}
+
+}  // namespace pct
+
+extern "C" int pct_kernel_timing(int enable) {
+    pct::KernelTimer& t = pct::g_timer;
+    std::lock_guard<std::mutex> lk(t.mu);
+    t.enabled = enable != 0;
+    if (t.enabled) t.used = 0;
+    return PCT_OK;
+}
+
+extern "C" int pct_kernel_timing_read(double* total_ms, int* launches) {
+    if (!total_ms || !launches) return PCT_ERR_INVALID;
+    pct::KernelTimer& t = pct::g_timer;
+    std::lock_guard<std::mutex> lk(t.mu);
+    double sum = 0;
+    for (size_t i = 0; i < t.used; ++i) {
+        if (hipEventSynchronize(t.pool[i].second) != hipSuccess) return PCT_ERR_HIP;
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, t.pool[i].first, t.pool[i].second) != hipSuccess)
+            return PCT_ERR_HIP;
+        sum += ms;
+    }
+    *total_ms = sum;
+    *launches = (int)t.used;
+    return PCT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI (GPU part)
+// ---------------------------------------------------------------------------
+
+using namespace pct;
+
+extern "C" int pct_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+extern "C" int pct_set_device(int device) {
+    if (device < 0 || device >= pct_device_count()) return PCT_ERR_NO_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) {
+        (void)hipGetLastError();
+        return PCT_ERR_NO_DEVICE;
+    }
+    return check_device(nullptr);
+}
+
+extern "C" int pct_normalize_labels_dev(float* d_label, size_t n, pct_stream_t stream) {
+    if (!d_label && n) return PCT_ERR_INVALID;
+    int rc = check_device(nullptr);
+    if (rc) return rc;
+    if (n == 0) return PCT_OK;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(normalize_labels_kernel, dim3(blocks), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), d_label, (long long)n);
+    return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;
+}
+
+extern "C" size_t pct_fuse_workspace_size(const pct_problem* pb, int choice) {
+    if (validate_problem(pb) != PCT_OK) return 0;
+    if (!is_tiled_choice(pb, choice)) return 0;
+    int sid = 0, grp = 1;
+    TileGeometry g;
+    if (!choose(pb, &sid, &grp, nullptr, &g)) return 0;
+    return tiled_ws_bytes(pb, g.n_groups);
+}
+
+static int fuse_dev_impl(const pct_problem* pb, const float* img, const float* atlas,
+                         const float* label, float* out, int choice, void* ws, size_t ws_bytes,
+                         bool own_ws, hipStream_t stream) {
+    int rc = validate_problem(pb);
+    if (rc) return rc;
+    if (!img || !atlas || !label || !out) return PCT_ERR_INVALID;
+    if (choice != PCT_CHOICE_GM && choice != PCT_CHOICE_GM2 && choice != PCT_CHOICE_SM)
+        return PCT_ERR_INVALID;
+    int dev = 0;
+    rc = check_device(&dev);
+    if (rc) return rc;
+    if (!is_tiled_choice(pb, choice)) {
+        if (choice == PCT_CHOICE_SM && (pb->nh - 1) / 2 > kMaxTiledHalfWindow)
+            return launch_direct(pb, img, atlas, label, out, stream);
+        return launch_direct(pb, img, atlas, label, out, stream);
+    }
+    int sid = 0, grp = 1;
+    TileGeometry g;
+    if (!choose(pb, &sid, &grp, nullptr, &g)) return PCT_ERR_UNSUPPORTED;
+    const size_t need = tiled_ws_bytes(pb, g.n_groups);
+    if (own_ws) {
+        rc = ensure_workspace(dev, need, &ws);
+        if (rc) return rc;
+        ws_bytes = need;
+    } else if (!ws || ws_bytes < need || (reinterpret_cast<uintptr_t>(ws) & 255)) {
+        return PCT_ERR_INVALID;
+    }
+    return launch_tiled(pb, img, atlas, label, out, ws, ws_bytes, sid, g, stream);
+}
+
+extern "C" int pct_fuse_dev(const pct_problem* pb, const float* d_img, const float* d_atlas,
+                            const float* d_label, float* d_out, int choice, pct_stream_t stream) {
+    return fuse_dev_impl(pb, d_img, d_atlas, d_label, d_out, choice, nullptr, 0, true,
+                         reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int pct_fuse_dev_ws(const pct_problem* pb, const float* d_img, const float* d_atlas,
+                               const float* d_label, float* d_out, int choice, void* d_ws,
+                               size_t ws_bytes, pct_stream_t stream) {
+    return fuse_dev_impl(pb, d_img, d_atlas, d_label, d_out, choice, d_ws, ws_bytes, false,
+                         reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int pct_fuse_host(const pct_problem* pb, const float* img, const float* atlas,
+                             const float* label, float* out, int choice) {
+    int rc = validate_problem(pb);
+    if (rc) return rc;
+    if (!img || !atlas || !label || !out) return PCT_ERR_INVALID;
+    rc = check_device(nullptr);
+    if (rc) return rc;
+    const size_t N = (size_t)pb->nx * pb->ny * pb->nz;
+    const size_t NA = N * pb->n_atlas;
+    float *d_img = nullptr, *d_atlas = nullptr, *d_label = nullptr, *d_out = nullptr;
+    hipStream_t s = nullptr;
+    rc = PCT_OK;
+    // cu:1687-1699: device allocation and host->device copies per call
+    if (hipMalloc(&d_img, N * 4) != hipSuccess || hipMalloc(&d_atlas, NA * 4) != hipSuccess ||
+        hipMalloc(&d_label, NA * 4) != hipSuccess || hipMalloc(&d_out, N * 4) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = PCT_ERR_NO_MEMORY;
+    }
+    if (!rc && (hipStreamCreate(&s) != hipSuccess)) rc = PCT_ERR_HIP;
+    if (!rc && (hipMemcpyAsync(d_img, img, N * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(d_atlas, atlas, NA * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(d_label, label, NA * 4, hipMemcpyHostToDevice, s) != hipSuccess))
+        rc = PCT_ERR_HIP;
+    if (!rc) rc = fuse_dev_impl(pb, d_img, d_atlas, d_label, d_out, choice, nullptr, 0, true, s);
+    if (!rc && hipMemcpyAsync(out, d_out, N * 4, hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = PCT_ERR_HIP;
+    if (s) {
+        if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = PCT_ERR_HIP;
+        (void)hipStreamDestroy(s);
+    }
+    // cu:1744
+    if (d_img) (void)hipFree(d_img);
+    if (d_atlas) (void)hipFree(d_atlas);
+    if (d_label) (void)hipFree(d_label);
+    if (d_out) (void)hipFree(d_out);
+    return rc;
+}
+
+extern "C" int pct_tile_config_count(void) { return (int)shapes().size(); }
+
+static void fill_config(const pct_problem* pb, int sid, int grp, double cost,
+                        const TileGeometry& g, pct_tile_config* out) {
+    const TileShape& s = shapes()[sid];
+    out->id = sid;
+    out->seg_width = s.seg_width;
+    out->bx = s.bx;
+    out->bz = s.bz;
+    out->waves = s.waves;
+    out->atlas_group = g.group;
+    out->lds_bytes = g.lds_bytes;
+    out->items = g.n_items;
+    out->est_cost = cost;
+    out->measured_ms = -1;
+    (void)pb;
+    (void)grp;
+}
+
+extern "C" int pct_tile_config_choose(const pct_problem* pb, pct_tile_config* out) {
+    int rc = validate_problem(pb);
+    if (rc) return rc;
+    if (!out) return PCT_ERR_INVALID;
+    int sid = 0, grp = 1;
+    double cost = 0;
+    TileGeometry g;
+    if (!choose(pb, &sid, &grp, &cost, &g)) return PCT_ERR_UNSUPPORTED;
+    fill_config(pb, sid, grp, cost, g, out);
+    return PCT_OK;
+}
+
+extern "C" int pct_tile_config_force(int id, int atlas_group) {
+    if (id >= (int)shapes().size()) return PCT_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(g_cfg_mutex);
+    g_forced_id = id < 0 ? -1 : id;
+    g_forced_group = atlas_group;
+    return PCT_OK;
+}
+
+extern "C" int pct_autotune(const pct_problem* pb, const float* d_img, const float* d_atlas,
+                            const float* d_label, float* d_out, int reps, pct_stream_t stream,
+                            pct_tile_config* best) {
+    int rc = validate_problem(pb);
+    if (rc) return rc;
+    if (!d_img || !d_atlas || !d_label || !d_out) return PCT_ERR_INVALID;
+    int dev = 0;
+    rc = check_device(&dev);
+    if (rc) return rc;
+    if ((pb->nh - 1) / 2 < 1 || (pb->nh - 1) / 2 > kMaxTiledHalfWindow) return PCT_ERR_UNSUPPORTED;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (reps < 1) reps = 1;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return PCT_ERR_HIP;
+    double best_ms = -1;
+    int best_sid = -1, best_grp = 1;
+    TileGeometry best_g;
+    // candidate groups: 1, the model's choice per shape, and n
+    for (int sid = 0; sid < (int)shapes().size(); ++sid) {
+        std::vector<int> groups = {1, pb->n_atlas};
+        double bc = -1;
+        int bg = 1;
+        for (int grp = 1; grp <= pb->n_atlas; ++grp) {
+            double c = model_cost(pb, sid, grp, nullptr);
+            if (c >= 0 && (bc < 0 || c < bc * 0.999)) {
+                bc = c;
+                bg = grp;
+            }
+        }
+        if (bc < 0) continue;
+        groups.push_back(bg);
+        std::sort(groups.begin(), groups.end());
+        groups.erase(std::unique(groups.begin(), groups.end()), groups.end());
+        for (int grp : groups) {
+            TileGeometry g;
+            if (!make_geometry(pb, sid, grp, &g)) continue;
+            void* ws = nullptr;
+            const size_t need = tiled_ws_bytes(pb, g.n_groups);
+            rc = ensure_workspace(dev, need, &ws);
+            if (rc) break;
+            // warm-up
+            rc = launch_tiled(pb, d_img, d_atlas, d_label, d_out, ws, need, sid, g, s);
+            if (rc) break;
+            (void)hipEventRecord(e0, s);
+            for (int i = 0; i < reps && !rc; ++i)
+                rc = launch_tiled(pb, d_img, d_atlas, d_label, d_out, ws, need, sid, g, s);
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = PCT_ERR_HIP;
+            if (rc) break;
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms /= reps;
+            if (best_ms < 0 || ms < best_ms) {
+                best_ms = ms;
+                best_sid = sid;
+                best_grp = grp;
+                best_g = g;
+            }
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (best_sid < 0) return PCT_ERR_UNSUPPORTED;
+    {
+        std::lock_guard<std::mutex> lk(g_cfg_mutex);
+        g_tuned[ShapeKey{pb->nx, pb->ny, pb->nz, pb->n_atlas, pb->nh}] = {best_sid, best_grp};
+    }
+    if (best) {
+        fill_config(pb, best_sid, best_grp, model_cost(pb, best_sid, best_grp, nullptr), best_g,
+                    best);
+        best->measured_ms = best_ms;
+    }
+    return PCT_OK;
+}
+
+extern "C" void pct_release(void) {
+    std::lock_guard<std::mutex> lk(g_dev_mutex);
+    for (auto& kv : g_dev) {
+        if (kv.second.ws) {
+            int cur = 0;
+            (void)hipGetDevice(&cur);
+            (void)hipSetDevice(kv.first);
+            (void)hipDeviceSynchronize();
+            (void)hipFree(kv.second.ws);
+            (void)hipSetDevice(cur);
+            kv.second.ws = nullptr;
+            kv.second.ws_bytes = 0;
+        }
+    }
+}

@@ -1,0 +1,421 @@
+// Host-side part of libpct.so: pre/post-processing around the fusion hot
+// path, the MatrixIO text format, the configuration file of the reference
+// program, and seeded input generators.  None of this computes the fusion.
+#include <algorithm>
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "pct_internal.h"
+
+namespace pct {
+
+uint64_t splitmix64(uint64_t& state) {
+    uint64_t z = (state += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+namespace {
+
+// Counter-based stream: value number `i` of stream `key`.  Independent of
+// evaluation order, so generation can be parallel and stays bit-identical.
+inline uint64_t hash_u64(uint64_t key, uint64_t i) {
+    uint64_t s = key ^ (i * 0xD1B54A32D192ED03ULL);
+    return splitmix64(s);
+}
+
+inline double uniform01(uint64_t key, uint64_t i) {  // [0, 1)
+    return (double)(hash_u64(key, i) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+inline double normal01(uint64_t key, uint64_t i) {  // Box-Muller, one variate per index
+    const double u1 = ((double)(hash_u64(key, 2 * i) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    const double u2 = (double)(hash_u64(key, 2 * i + 1) >> 11) * (1.0 / 9007199254740992.0);
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+inline uint64_t derive_key(uint64_t seed, uint64_t a, uint64_t b) {
+    uint64_t s = seed ^ (0xA0761D6478BD642FULL * (a + 1)) ^ (0xE7037ED1A0B428DBULL * (b + 1));
+    splitmix64(s);
+    return splitmix64(s);
+}
+
+inline int mirror(int i, int n) {  // symmetric reflection without edge repeat
+    if (n == 1) return 0;
+    const int period = 2 * n - 2;
+    i %= period;
+    if (i < 0) i += period;
+    return i < n ? i : period - i;
+}
+
+// Separable Gaussian blur, sigma 3 voxels, taps to 4 sigma, in place.
+void gaussian_blur(std::vector<double>& f, int nx, int ny, int nz) {
+    const double sigma = 3.0;
+    const int R = 12;
+    double taps[2 * R + 1];
+    double sum = 0;
+    for (int k = -R; k <= R; ++k) sum += taps[k + R] = std::exp(-(double)(k * k) / (2 * sigma * sigma));
+    for (double& t : taps) t /= sum;
+    std::vector<double> tmp(f.size());
+    const long long pl = (long long)nx * ny;
+    // y
+#pragma omp parallel for schedule(static)
+    for (long long row = 0; row < (long long)nz * nx; ++row)
+        for (int y = 0; y < ny; ++y) {
+            double acc = 0;
+            for (int k = -R; k <= R; ++k) acc += taps[k + R] * f[row * ny + mirror(y + k, ny)];
+            tmp[row * ny + y] = acc;
+        }
+    // x
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x)
+            for (int y = 0; y < ny; ++y) {
+                double acc = 0;
+                for (int k = -R; k <= R; ++k)
+                    acc += taps[k + R] * tmp[z * pl + (long long)mirror(x + k, nx) * ny + y];
+                f[z * pl + (long long)x * ny + y] = acc;
+            }
+    // z
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < nz; ++z)
+        for (long long xy = 0; xy < pl; ++xy) {
+            double acc = 0;
+            for (int k = -R; k <= R; ++k) acc += taps[k + R] * f[mirror(z + k, nz) * pl + xy];
+            tmp[z * pl + xy] = acc;
+        }
+    f.swap(tmp);
+}
+
+// Tissue classes 0..3 from blurred noise thresholded at its quartiles.
+std::vector<unsigned char> tissue_classes(uint64_t key, int nx, int ny, int nz) {
+    const long long n = (long long)nx * ny * nz;
+    std::vector<double> f(n);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n; ++i) f[i] = normal01(key, (uint64_t)i);
+    gaussian_blur(f, nx, ny, nz);
+    std::vector<double> s(f);
+    auto kth = [&](long long k) {
+        std::nth_element(s.begin(), s.begin() + k, s.end());
+        return s[k];
+    };
+    const double q25 = kth(n / 4), q50 = kth(n / 2), q75 = kth((3 * n) / 4);
+    std::vector<unsigned char> c(n);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n; ++i)
+        c[i] = (unsigned char)((f[i] > q25) + (f[i] > q50) + (f[i] > q75));
+    return c;
+}
+
+const double kMriMean[4] = {0.0, 0.8, 0.5, 0.2};
+const double kCtMean[4] = {-1000.0, -100.0, 40.0, 1000.0};
+
+void fill_intensities(const std::vector<unsigned char>& cls, uint64_t key, long long n, float* mri,
+                      float* ct) {
+    const uint64_t kmri = derive_key(key, 101, 0), kct = derive_key(key, 102, 0);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n; ++i) {
+        const int c = cls[i];
+        if (mri) mri[i] = (float)(kMriMean[c] + 0.03 * normal01(kmri, (uint64_t)i));
+        if (ct) ct[i] = (float)(kCtMean[c] + 20.0 * normal01(kct, (uint64_t)i));
+    }
+}
+
+}  // namespace
+}  // namespace pct
+
+using namespace pct;
+
+extern "C" const char* pct_strerror(int status) {
+    switch (status) {
+        case PCT_OK: return "ok";
+        case PCT_ERR_INVALID: return "invalid argument";
+        case PCT_ERR_NO_DEVICE: return "no gfx950 HIP device available";
+        case PCT_ERR_HIP: return "HIP runtime error";
+        case PCT_ERR_NO_MEMORY: return "out of memory";
+        case PCT_ERR_IO: return "file could not be opened or written";
+        case PCT_ERR_FORMAT: return "malformed file";
+        case PCT_ERR_DIMS: return "dimensions do not match";
+        case PCT_ERR_UNSUPPORTED: return "no kernel configuration fits the request";
+        case PCT_ERR_TOO_LARGE: return "request exceeds a documented maximum";
+        default: return "unknown status";
+    }
+}
+
+extern "C" int pct_abi_version(void) { return PCT_ABI_VERSION; }
+
+extern "C" double pct_weight_divisor(double sigma, double beta) {
+    // cpp:1601 - (2 * 27 * beta*pow(sigma, 2)), evaluated left to right
+    return (2 * 27 * beta * std::pow(sigma, 2));
+}
+
+extern "C" void pct_normalize_labels(float* label, size_t n) {
+    if (!label) return;
+    for (size_t i = 0; i < n; ++i) label[i] = 255.0f * ((label[i] + 1024.0f) / 4095.0f);
+}
+
+// ---- post-processing ------------------------------------------------------
+
+namespace {
+// Median of the non-NaN values (cpp:1020-1043): stable sort, middle value or
+// float64 mean of the two middle values.  NaN when every value is NaN.
+double median27(const float* patch) {
+    float v[27];
+    int cnt = 0;
+    for (int j = 0; j < 27; ++j)
+        if (!std::isnan(patch[j])) v[cnt++] = patch[j];
+    if (cnt == 0) return NAN;
+    std::stable_sort(v, v + cnt);
+    if (cnt % 2 == 0) return ((double)v[cnt / 2 - 1] + (double)v[cnt / 2]) / 2;
+    return (double)v[cnt / 2];
+}
+}  // namespace
+
+extern "C" int pct_regularize(int nh, int nx, int ny, int nz, float* arr) {
+    if (!arr || nh < 1 || nx < 1 || ny < 1 || nz < 1) return PCT_ERR_INVALID;
+    const int hp = 1, hw = (nh - 1) / 2, lo = hw + hp;
+    const long long pl = (long long)nx * ny;
+    int passes = 0;
+    bool again = true;
+    while (again) {
+        again = false;
+        long long filled = 0;
+        ++passes;
+        for (int z = lo; z < nz - lo; ++z)
+            for (int x = lo; x < nx - lo; ++x)
+                for (int y = lo; y < ny - lo; ++y) {
+                    const long long id = z * pl + (long long)x * ny + y;
+                    if (!std::isnan(arr[id])) continue;
+                    float patch[27];
+                    int c = 0;
+                    for (int r = -1; r <= 1; ++r)
+                        for (int s = -1; s <= 1; ++s)
+                            for (int t = -1; t <= 1; ++t)
+                                patch[c++] = arr[id + r * pl + (long long)s * ny + t];
+                    arr[id] = (float)median27(patch);
+                    if (std::isnan(arr[id]))
+                        again = true;
+                    else
+                        ++filled;
+                }
+        if (again && filled == 0) break;
+    }
+    return passes;
+}
+
+extern "C" void pct_round(float* arr, size_t n) {
+    if (!arr) return;
+    for (size_t i = 0; i < n; ++i) arr[i] = std::round(arr[i]);
+}
+
+extern "C" void pct_to_hu(float* arr, size_t n) {
+    if (!arr) return;
+    const float max_labels = 4095.0f, min_labels = 1024.0f;
+    for (size_t i = 0; i < n; ++i) arr[i] = std::round(max_labels * (arr[i] / 255.0f)) - min_labels;
+}
+
+extern "C" int pct_postprocess(int nh, int nx, int ny, int nz, float* arr) {
+    int rc = pct_regularize(nh, nx, ny, nz, arr);
+    if (rc < 0) return rc;
+    const size_t n = (size_t)nx * ny * nz;
+    pct_round(arr, n);
+    pct_to_hu(arr, n);
+    return PCT_OK;
+}
+
+// ---- MatrixIO --------------------------------------------------------------
+
+extern "C" void pct_free(void* p) { std::free(p); }
+
+extern "C" int pct_matrixio_read(const char* path, float** data, int dims4[4]) {
+    if (!path || !data || !dims4) return PCT_ERR_INVALID;
+    *data = nullptr;
+    std::ifstream in(path);
+    if (!in.is_open()) return PCT_ERR_IO;
+    std::string line;
+    long long hdr[4];
+    for (int i = 0; i < 4; ++i) {
+        if (!std::getline(in, line)) return PCT_ERR_FORMAT;
+        hdr[i] = std::atoll(line.c_str());
+    }
+    const long long num = hdr[0], rows = hdr[1], cols = hdr[2], depth = hdr[3];
+    if (num <= 0 || rows <= 0 || cols <= 0 || depth <= 0) return PCT_ERR_FORMAT;
+    const long long size = num * rows * cols * depth;
+    if (size > (1LL << 34)) return PCT_ERR_TOO_LARGE;
+    float* v = static_cast<float*>(std::malloc(sizeof(float) * (size_t)size));
+    if (!v) return PCT_ERR_NO_MEMORY;
+    long long idx = 0;
+    // values separated by blanks/newlines, in memory order (cpp:273-285)
+    std::string tok;
+    while (in >> tok) {
+        errno = 0;
+        char* end = nullptr;
+        const float f = std::strtof(tok.c_str(), &end);
+        if (end == tok.c_str() || *end != '\0') {
+            std::free(v);
+            return PCT_ERR_FORMAT;
+        }
+        if (idx >= size) {
+            std::free(v);
+            return PCT_ERR_FORMAT;
+        }
+        v[idx++] = f;
+    }
+    if (idx != size) {
+        std::free(v);
+        return PCT_ERR_FORMAT;
+    }
+    dims4[0] = (int)rows;
+    dims4[1] = (int)cols;
+    dims4[2] = (int)depth;
+    dims4[3] = (int)num;
+    *data = v;
+    return PCT_OK;
+}
+
+extern "C" int pct_matrixio_write(const char* path, const float* data, int num, int rows, int cols,
+                                  int depth) {
+    if (!path || (!data && (long long)num * rows * cols * depth > 0)) return PCT_ERR_INVALID;
+    if (num < 0 || rows < 0 || cols < 0 || depth < 0) return PCT_ERR_INVALID;
+    FILE* f = std::fopen(path, "w");
+    if (!f) return PCT_ERR_IO;
+    const long long size = (long long)num * rows * cols * depth;
+    std::fprintf(f, "%d\n%d\n%d\n%d\n", num, rows, cols, depth);
+    std::vector<char> buf(1 << 16);
+    size_t used = 0;
+    for (long long i = 0; i < size; ++i) {
+        if (buf.size() - used < 64) {
+            std::fwrite(buf.data(), 1, used, f);
+            used = 0;
+        }
+        // std::ostream << float with the default precision 6 (cpp:307)
+        used += (size_t)std::snprintf(buf.data() + used, 64, "%g ", (double)data[i]);
+    }
+    std::fwrite(buf.data(), 1, used, f);
+    const bool ok = std::ferror(f) == 0;
+    return (std::fclose(f) == 0 && ok) ? PCT_OK : PCT_ERR_IO;
+}
+
+// ---- configuration file and generators ----------------------------------
+
+extern "C" void pct_volume_set_free(pct_volume_set* vs) {
+    if (!vs) return;
+    std::free(vs->img);
+    std::free(vs->atlas);
+    std::free(vs->label);
+    vs->img = vs->atlas = vs->label = nullptr;
+}
+
+extern "C" int pct_synth_uniform(int nx, int ny, int nz, int n_atlas, uint64_t seed, float* img,
+                                 float* atlas, float* label) {
+    if (nx < 1 || ny < 1 || nz < 1 || n_atlas < 1 || !img || !atlas || !label)
+        return PCT_ERR_INVALID;
+    const long long n = (long long)nx * ny * nz;
+    const long long na = n * n_atlas;
+    const uint64_t ka = derive_key(seed, 1, 0), kl = derive_key(seed, 2, 0),
+                   ki = derive_key(seed, 3, 0);
+    // cpp:961-973: 10*U values for atlas and label, then the image
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < na; ++i) {
+        atlas[i] = 10.0f * (float)uniform01(ka, (uint64_t)i);
+        label[i] = 10.0f * (float)uniform01(kl, (uint64_t)i);
+    }
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n; ++i) img[i] = 10.0f * (float)uniform01(ki, (uint64_t)i);
+    return PCT_OK;
+}
+
+extern "C" int pct_synth_tissue(int nx, int ny, int nz, int n_atlas, int max_shift,
+                                int independent_anatomy, uint64_t seed, float* target_mri,
+                                float* atlas_mri, float* atlas_ct) {
+    if (nx < 1 || ny < 1 || nz < 1 || n_atlas < 0 || max_shift < 0) return PCT_ERR_INVALID;
+    if (!target_mri || (n_atlas > 0 && (!atlas_mri || !atlas_ct))) return PCT_ERR_INVALID;
+    const long long n = (long long)nx * ny * nz;
+    const long long pl = (long long)nx * ny;
+    std::vector<unsigned char> cls = tissue_classes(derive_key(seed, 0, 0), nx, ny, nz);
+    fill_intensities(cls, derive_key(seed, 0, 1), n, target_mri, nullptr);
+    for (int a = 0; a < n_atlas; ++a) {
+        std::vector<unsigned char> acls;
+        if (independent_anatomy) {
+            acls = tissue_classes(derive_key(seed, (uint64_t)a + 1, 0), nx, ny, nz);
+        } else {
+            const uint64_t ks = derive_key(seed, (uint64_t)a + 1, 2);
+            const int span = 2 * max_shift + 1;
+            const int sz = (int)(hash_u64(ks, 0) % (uint64_t)span) - max_shift;
+            const int sx = (int)(hash_u64(ks, 1) % (uint64_t)span) - max_shift;
+            const int sy = (int)(hash_u64(ks, 2) % (uint64_t)span) - max_shift;
+            acls.resize(n);
+#pragma omp parallel for schedule(static)
+            for (int z = 0; z < nz; ++z)
+                for (int x = 0; x < nx; ++x)
+                    for (int y = 0; y < ny; ++y) {
+                        const int zz = std::min(std::max(z + sz, 0), nz - 1);
+                        const int xx = std::min(std::max(x + sx, 0), nx - 1);
+                        const int yy = std::min(std::max(y + sy, 0), ny - 1);
+                        acls[z * pl + (long long)x * ny + y] = cls[zz * pl + (long long)xx * ny + yy];
+                    }
+        }
+        fill_intensities(acls, derive_key(seed, (uint64_t)a + 1, 1), n, atlas_mri + a * n,
+                         atlas_ct + a * n);
+    }
+    return PCT_OK;
+}
+
+extern "C" int pct_read_config(const char* path, uint64_t seed, pct_volume_set* out) {
+    if (!path || !out) return PCT_ERR_INVALID;
+    std::memset(out, 0, sizeof(*out));
+    std::ifstream in(path);
+    if (!in.is_open()) return PCT_ERR_IO;
+    // cpp:922-958: a line-indexed state machine
+    std::string line, atlas_path, label_path, img_path;
+    int state = 0, random_data = 0, num = 0, nx = 0, ny = 0, nz = 0;
+    while (std::getline(in, line)) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        switch (state) {
+            case 0: random_data = std::atoi(line.c_str()); state++; break;
+            case 1: if (random_data) num = std::atoi(line.c_str()); else atlas_path = line; state++; break;
+            case 2: if (random_data) nx = std::atoi(line.c_str()); else label_path = line; state++; break;
+            case 3: if (random_data) ny = std::atoi(line.c_str()); else img_path = line; state++; break;
+            case 4: if (random_data) { nz = std::atoi(line.c_str()); state++; } break;
+            default: break;
+        }
+    }
+    if (random_data) {
+        if (state < 5 || num < 1 || nx < 1 || ny < 1 || nz < 1) return PCT_ERR_FORMAT;
+        const size_t n = (size_t)nx * ny * nz;
+        out->nx = nx; out->ny = ny; out->nz = nz; out->n_atlas = num;
+        out->img = static_cast<float*>(std::malloc(n * 4));
+        out->atlas = static_cast<float*>(std::malloc(n * num * 4));
+        out->label = static_cast<float*>(std::malloc(n * num * 4));
+        if (!out->img || !out->atlas || !out->label) {
+            pct_volume_set_free(out);
+            return PCT_ERR_NO_MEMORY;
+        }
+        return pct_synth_uniform(nx, ny, nz, num, seed, out->img, out->atlas, out->label);
+    }
+    if (state < 4) return PCT_ERR_FORMAT;
+    int da[4], dl[4], di[4];
+    float *a = nullptr, *l = nullptr, *im = nullptr;
+    int rc = pct_matrixio_read(atlas_path.c_str(), &a, da);
+    if (!rc) rc = pct_matrixio_read(label_path.c_str(), &l, dl);
+    if (!rc) rc = pct_matrixio_read(img_path.c_str(), &im, di);
+    // cpp:991-1007: atlas and label dims equal, image dims equal in x, y, z
+    if (!rc && !(da[0] == dl[0] && da[1] == dl[1] && da[2] == dl[2] && da[3] == dl[3] &&
+                 da[0] == di[0] && da[1] == di[1] && da[2] == di[2]))
+        rc = PCT_ERR_DIMS;
+    if (rc) {
+        std::free(a); std::free(l); std::free(im);
+        return rc;
+    }
+    out->nx = dl[0]; out->ny = dl[1]; out->nz = dl[2]; out->n_atlas = dl[3];
+    out->img = im; out->atlas = a; out->label = l;
+    return PCT_OK;
+}

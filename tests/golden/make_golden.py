@@ -1,0 +1,154 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures of tests/golden/ from the compiled reference.
+
+The reference repository ships no tests, fixtures or golden vectors, so the
+vectors here come from the reference CPU program itself
+(src/CPU/NewPatchedBrainPureC.cpp), compiled from its source into
+oracle/_ref/libpctref.so by oracle/Makefile.  Inputs come from the seeded
+generators of libpct (pct_synth_tissue / pct_synth_uniform); every fixture
+records a sha256 of its inputs so a changed generator is detected.
+
+Fixtures
+  small_tissue.npz    16x18x14 tissue phantom, n=2, NH=5, sigma 2.5, beta 1:
+                      inputs, fused value (MultiAtlasPatchBasedSegmentationOpenMP)
+                      and final HU after RegularizationStep/RoundMatrix/rescale.
+  small_uniform.npz   random-mode data (10*U[0,1)), 14x15x13, n=2, NH=3,
+                      sigma 0.33, beta 1: a mix of NaN and finite voxels, so the
+                      NaN rule and the median fill are exercised.
+  cfg1.json           64x64x32, n=2, NH=7: input and
+                      output sha256 plus 2048 sampled voxel values.
+  cfg2_sample.npz     222x222x112, n=5, NH=9 and
+  cfg3_sample.npz     222x222x112, n=10, NH=11: input sha256, sampled voxel
+                      indices, the reference's float32 value at each (computed
+                      on crops that contain the voxel's whole neighbourhood)
+                      and the float64 oracle value P/S.
+
+Usage: python tests/golden/make_golden.py   (needs oracle/_ref and libpct.so)
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+import pseudoctimaging_amd as pct  # noqa: E402
+
+
+def digest(*arrays) -> str:
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def ref_pipeline(img, amri, act_hu, nh, sigma, beta):
+    """main()'s chain with the reference's own functions: normalisation
+    (cpp:1583-1593, inline in main, restated), fusion, RegularizationStep,
+    RoundMatrix, HU rescale (cpp:1733-1738, inline in main, restated)."""
+    lab = act_hu.copy()
+    lab = 255.0 * ((lab + np.float32(1024.0)) / np.float32(4095.0))
+    lab = lab.astype(np.float32)
+    lab2 = act_hu.copy()
+    oracle.normalize_labels(lab2)
+    assert np.array_equal(lab.view(np.uint32), lab2.view(np.uint32))
+    fused = oracle.ref_fuse(img, amri, lab2, nh, sigma, beta, threads=8)
+    hu = fused.copy()
+    oracle.ref_regularize(hu, nh)
+    oracle.ref_round(hu)
+    hu = (np.round(np.float32(4095.0) * (hu / np.float32(255.0))) - np.float32(1024.0)) \
+        .astype(np.float32)
+    return lab2, fused, hu
+
+
+def small_tissue():
+    nz, nx, ny, n, nh, sigma, beta, seed = 14, 16, 18, 2, 5, 2.5, 1.0, 101
+    img, amri, act = pct.synth_tissue(nx, ny, nz, n, 2, False, seed)
+    lab, fused, hu = ref_pipeline(img, amri, act, nh, sigma, beta)
+    np.savez_compressed(os.path.join(HERE, "small_tissue.npz"), img=img, atlas=amri, ct_hu=act,
+                        label=lab, fused=fused, hu=hu,
+                        params=np.array([nx, ny, nz, n, nh, sigma, beta, seed]))
+    print("small_tissue", fused.shape, int(np.isnan(fused).sum()), "NaN")
+
+
+def small_uniform():
+    nz, nx, ny, n, nh, sigma, beta, seed = 13, 14, 15, 2, 3, 0.33, 1.0, 202
+    img, atlas, label = pct.synth_uniform(nx, ny, nz, n, seed)
+    lab, fused, hu = ref_pipeline(img, atlas, label, nh, sigma, beta)
+    np.savez_compressed(os.path.join(HERE, "small_uniform.npz"), img=img, atlas=atlas,
+                        ct_hu=label, label=lab, fused=fused, hu=hu,
+                        params=np.array([nx, ny, nz, n, nh, sigma, beta, seed]))
+    r = (nh - 1) // 2 + 1
+    inner = fused[r:-r, r:-r, r:-r]
+    print("small_uniform: NaN", int(np.isnan(inner).sum()), "of", inner.size)
+
+
+def cfg1():
+    nz, nx, ny, n, nh, sigma, beta, seed = 32, 64, 64, 2, 7, 2.5, 1.0, 1
+    img, amri, act = pct.synth_tissue(nx, ny, nz, n, 2, False, seed)
+    lab, fused, hu = ref_pipeline(img, amri, act, nh, sigma, beta)
+    rng = np.random.default_rng(11)
+    idx = rng.choice(fused.size, 2048, replace=False)
+    out = {
+        "shape_zxy": [nz, nx, ny], "n_atlas": n, "nh": nh, "sigma": sigma, "beta": beta,
+        "seed": seed, "max_shift": 2,
+        "input_sha256": digest(img, amri, act),
+        "fused_sha256": digest(fused), "hu_sha256": digest(hu),
+        "sample_index": idx.tolist(),
+        "sample_fused": [float(v) for v in fused.reshape(-1)[idx]],
+        "sample_hu": [float(v) for v in hu.reshape(-1)[idx]],
+    }
+    with open(os.path.join(HERE, "cfg1.json"), "w") as f:
+        json.dump(out, f)
+    print("cfg1 written")
+
+
+def sampled(name, nz, nx, ny, n, nh, seed, centres=96, block=4):
+    sigma, beta = 2.5, 1.0
+    img, amri, act = pct.synth_tissue(nx, ny, nz, n, 3, False, seed)
+    lab = act.copy()
+    oracle.normalize_labels(lab)
+    r = (nh - 1) // 2 + 1
+    rng = np.random.default_rng(seed + 1)
+    idx_all, ref_all = [], []
+    for _ in range(centres):
+        # a block x block x block group of voxels starting at (z, x, y)
+        z = int(rng.integers(r, nz - r - block + 1))
+        x = int(rng.integers(r, nx - r - block + 1))
+        y = int(rng.integers(r, ny - r - block + 1))
+        zs, xs, ys = z - r, x - r, y - r
+        crop = (slice(zs, z + block + r), slice(xs, x + block + r), slice(ys, y + block + r))
+        ci = np.ascontiguousarray(img[crop])
+        ca = np.ascontiguousarray(amri[(slice(None),) + crop])
+        cl = np.ascontiguousarray(lab[(slice(None),) + crop])
+        out = oracle.ref_fuse(ci, ca, cl, nh, sigma, beta, threads=8)
+        got = out[r:r + block, r:r + block, r:r + block]
+        zz, xx, yy = np.meshgrid(np.arange(z, z + block), np.arange(x, x + block),
+                                 np.arange(y, y + block), indexing="ij")
+        idx_all.append(((zz * nx + xx) * ny + yy).reshape(-1))
+        ref_all.append(got.reshape(-1))
+    idx = np.concatenate(idx_all).astype(np.int64)
+    ref = np.concatenate(ref_all).astype(np.float32)
+    fp64 = oracle.voxels(img, amri, lab, nh, sigma, beta, idx)
+    np.savez_compressed(os.path.join(HERE, f"{name}_sample.npz"), index=idx, ref_fused=ref,
+                        fp64=fp64,
+                        params=np.array([nx, ny, nz, n, nh, sigma, beta, seed, 3]),
+                        input_sha256=np.array(digest(img, amri, act)))
+    print(name, idx.size, "voxels")
+
+
+if __name__ == "__main__":
+    if not oracle.ref_available():
+        sys.exit("oracle/_ref/libpctref.so is required (make -C oracle)")
+    small_tissue()
+    small_uniform()
+    cfg1()
+    sampled("cfg2", 112, 222, 222, 5, 9, 2)
+    sampled("cfg3", 112, 222, 222, 10, 11, 3)

@@ -1,0 +1,68 @@
+"""GPU parity of the plain FP32 fusion (tiled SM kernel and direct GM/GM2
+kernel) against the oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle
+import pseudoctimaging_amd as pct
+from parity import compare
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(nx, ny, nz, n, shift=2, seed=0, independent=False):
+    img, amri, act = pct.synth_tissue(nx, ny, nz, n, shift, independent, seed)
+    lab = act.copy()
+    pct.normalize_labels(lab)
+    return img, amri, lab
+
+
+SMALL = [
+    # nx, ny, nz, n, nh, sigma, beta
+    (24, 26, 20, 2, 3, 2.5, 1.0),
+    (30, 34, 22, 2, 7, 2.5, 1.0),
+    (28, 40, 24, 3, 9, 2.5, 1.0),
+    (34, 30, 26, 2, 11, 2.5, 1.0),
+    (33, 37, 25, 1, 5, 1.0, 1.0),
+    (40, 70, 21, 2, 7, 2.5, 0.5),
+]
+
+
+@pytest.mark.parametrize("nx,ny,nz,n,nh,sigma,beta", SMALL)
+@pytest.mark.parametrize("choice", [pct.CHOICE_SM, pct.CHOICE_GM2])
+def test_plain_vs_oracle(gpu, nx, ny, nz, n, nh, sigma, beta, choice):
+    print(f"case {nx}x{ny}x{nz} n={n} nh={nh} choice={choice}", flush=True)
+    img, amri, lab = _case(nx, ny, nz, n, seed=nx * 7 + nh)
+    got = pct.multi_patch_based_segmentation_gpu(img, amri, lab, nh, sigma, beta, choice)
+    ref = oracle.fuse(img, amri, lab, nh, sigma, beta)
+    st = compare(got, ref)
+    # border voxels are exactly 0
+    r = (nh - 1) // 2 + 1
+    mask = np.ones(img.shape, bool)
+    mask[r:nz - r, r:nx - r, r:ny - r] = False
+    assert np.all(got[mask] == 0)
+    print(st)
+
+
+def test_all_tile_shapes_agree(gpu):
+    img, amri, lab = _case(44, 70, 30, 3, seed=3)
+    ref = oracle.fuse(img, amri, lab, 7, 2.5, 1.0)
+    outs = []
+    try:
+        for sid in range(pct._lib.lib().pct_tile_config_count()):
+            for grp in (1, 2, 3):
+                pct.force_tile_config(sid, grp)
+                got = pct.multi_patch_based_segmentation_gpu(img, amri, lab, 7, 2.5, 1.0,
+                                                             pct.CHOICE_SM)
+                compare(got, ref)
+                outs.append(got)
+    finally:
+        pct.force_tile_config(-1, 0)
+    assert len(outs) >= 3
+
+
+def test_deterministic(gpu):
+    img, amri, lab = _case(36, 66, 28, 2, seed=5)
+    a = pct.multi_patch_based_segmentation_gpu(img, amri, lab, 9, 2.5, 1.0)
+    b = pct.multi_patch_based_segmentation_gpu(img, amri, lab, 9, 2.5, 1.0)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
