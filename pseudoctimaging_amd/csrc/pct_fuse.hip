@@ -173,12 +173,70 @@ __device__ __forceinline__ void tiled_step(const float* __restrict__ lds, int ba
     }
 }
 
-template <int W, int BX, int BZ, int NW, int HW>
-__global__ __launch_bounds__(NW * 64) void fuse_tiled(TiledArgs args) {
-    using C = TiledCfg<W, BX, BZ, NW, HW>;
-    extern __shared__ float lds[];
+// Copies the slab of one (atlas, dz) stage into LDS: plane p < APLANES holds
+// the atlas MRI at z = zdz-1+p (scaled), plane APLANES+j the atlas CT at
+// z = zdz+j; row rr is volume x = xs+rr, column c is volume y = ys+c; zero
+// outside the volume.  Each wave issues the loads of all its rows of two
+// planes before it writes any of them, so global latency is paid once per
+// pair of planes.
+template <class C, int NW>
+__device__ __forceinline__ void stage_slab(float* __restrict__ lds, const float* __restrict__ A,
+                                           const float* __restrict__ L, float scale, int zdz,
+                                           int xs, int ys, int nx, int ny, int nz,
+                                           long long plane, int wv, int lane) {
+    constexpr int NP = C::APLANES + C::LPLANES;
+    constexpr int RPW = (C::RX + NW - 1) / NW;   // rows per wave and plane
+    constexpr int PB = 2;                         // planes per batch
+    static_assert(NP % PB == 0, "plane batches must tile the slab");
+#pragma unroll 1
+    for (int p0 = 0; p0 < NP; p0 += PB) {
+        float v[PB][RPW][C::COLCHUNKS];
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            const int p = p0 + q;
+            const bool is_mri = p < C::APLANES;
+            const int zz = is_mri ? zdz - 1 + p : zdz + (p - C::APLANES);
+            const float* __restrict__ src = is_mri ? A : L;
+            const float mul = is_mri ? scale : 1.f;
+            const bool zok = zz >= 0 && zz < nz;
+            const long long zoff = (long long)zz * plane;
+#pragma unroll
+            for (int k = 0; k < RPW; ++k) {
+                const int rr = wv + k * NW;
+                const int xx = xs + rr;
+                const bool xok = zok && rr < C::RX && xx >= 0 && xx < nx;
+#pragma unroll
+                for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
+                    const int c = lane + cc * 64;
+                    const int yy = ys + c;
+                    float t = 0.f;
+                    if (xok && c < C::CW && yy >= 0 && yy < ny)
+                        t = src[zoff + (long long)xx * ny + yy] * mul;
+                    v[q][k][cc] = t;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < PB; ++q)
+#pragma unroll
+            for (int k = 0; k < RPW; ++k) {
+                const int rr = wv + k * NW;
+#pragma unroll
+                for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
+                    const int c = lane + cc * 64;
+                    if (rr < C::RX && c < C::CW) lds[(p0 + q) * C::PS + rr * C::CW + c] = v[q][k][cc];
+                }
+            }
+    }
+}
 
-    const int item = xcd_contiguous(blockIdx.x, args.n_items);
+// One work item (tile, atlas group) of one problem: the whole workgroup runs
+// it.  Shared by the single-problem kernel and the persistent batched kernel,
+// so a problem gives bit-identical results through either.
+template <int W, int BX, int BZ, int NW, int HW>
+__device__ __forceinline__ void tiled_item(const TiledArgs& args, int item,
+                                           float* __restrict__ lds) {
+    using C = TiledCfg<W, BX, BZ, NW, HW>;
     const int g = item / args.n_tiles;
     int t = item - g * args.n_tiles;
     const int ty = t % args.tiles_y;
@@ -231,32 +289,7 @@ __global__ __launch_bounds__(NW * 64) void fuse_tiled(TiledArgs args) {
         const float* __restrict__ L = args.label + (long long)a * args.N;
         for (int dz = -HW; dz <= HW; ++dz) {
             __syncthreads();
-            // Stage: LDS plane p < APLANES holds atlas MRI at z = z0+dz-1+p
-            // (scaled), plane APLANES+j holds atlas CT at z = z0+dz+j.
-#pragma unroll 1
-            for (int p = 0; p < C::APLANES + C::LPLANES; ++p) {
-                const bool is_mri = p < C::APLANES;
-                const int zz = is_mri ? z0 + dz - 1 + p : z0 + dz + (p - C::APLANES);
-                const float* __restrict__ src = is_mri ? A : L;
-                const float mul = is_mri ? args.scale : 1.f;
-                const bool zok = zz >= 0 && zz < nz;
-                const long long zoff = (long long)zz * plane;
-#pragma unroll 4
-                for (int rr = wv; rr < C::RX; rr += NW) {
-                    const int xx = xs + rr;
-                    const bool xok = zok && xx >= 0 && xx < nx;
-#pragma unroll
-                    for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
-                        const int c = lane + cc * 64;
-                        const int yy = ys + c;
-                        if (c < C::CW) {
-                            float v = 0.f;
-                            if (xok && yy >= 0 && yy < ny) v = src[zoff + (long long)xx * ny + yy] * mul;
-                            lds[p * C::PS + rr * C::CW + c] = v;
-                        }
-                    }
-                }
-            }
+            stage_slab<C, NW>(lds, A, L, args.scale, z0 + dz, xs, ys, nx, ny, nz, plane, wv, lane);
             __syncthreads();
 #pragma unroll 1
             for (int dx = 0; dx <= 2 * HW; ++dx) {
@@ -284,6 +317,12 @@ __global__ __launch_bounds__(NW * 64) void fuse_tiled(TiledArgs args) {
                 }
             }
     }
+}
+
+template <int W, int BX, int BZ, int NW, int HW>
+__global__ __launch_bounds__(NW * 64) void fuse_tiled(TiledArgs args) {
+    extern __shared__ float lds[];
+    tiled_item<W, BX, BZ, NW, HW>(args, xcd_contiguous(blockIdx.x, args.n_items), lds);
 }
 
 // Sum of the per-group partials in group order; border voxels get 0
@@ -374,12 +413,15 @@ struct TiledEntry {
     int hw;
     TiledFn fn;
     int lds_bytes;
+    // resident workgroups per CU reported by the runtime (registers and LDS);
+    // 0 until a gfx950 device has been seen
+    mutable int occupancy;
 };
 
 template <int W, int BX, int BZ, int NW, int HW>
 TiledEntry make_entry() {
     using C = TiledCfg<W, BX, BZ, NW, HW>;
-    return TiledEntry{{W, BX, BZ, NW}, HW, &fuse_tiled<W, BX, BZ, NW, HW>, C::LDS_BYTES};
+    return TiledEntry{{W, BX, BZ, NW}, HW, &fuse_tiled<W, BX, BZ, NW, HW>, C::LDS_BYTES, 0};
 }
 
 template <int W, int BX, int BZ, int NW>
@@ -401,6 +443,16 @@ static const std::vector<TileShape>& shapes() {
         {32, 4, 4, 2},   // 2: small LDS footprint, several workgroups per CU
         {32, 2, 8, 4},   // 3: 16 x rows, 8 planes
         {64, 4, 4, 4},   // 4: 16 x rows, 4 planes
+        {32, 4, 4, 4},   // 5: 32 x rows, 4 planes; two workgroups per CU up to NH=11
+        {32, 2, 6, 4},   // 6: 16 x rows, 6 planes; two workgroups per CU up to NH=11
+        {32, 2, 4, 4},   // 7: 16 x rows, 4 planes; three workgroups per CU up to NH=11
+        {32, 4, 2, 4},   // 8: 32 x rows, 2 planes; three workgroups per CU up to NH=11
+        {64, 2, 4, 4},   // 9: 8 x rows of 62-wide segments, 4 planes
+        {32, 2, 2, 8},   // 10: 32 x rows, 2 planes, eight waves share one slab
+        {32, 2, 2, 4},   // 11: 16 x rows, 2 planes; five workgroups per CU up to NH=11
+        {32, 2, 2, 2},   // 12: 8 x rows, 2 planes, two waves; eight workgroups per CU
+        {32, 2, 4, 8},   // 13: 32 x rows, 4 planes, eight waves
+        {32, 4, 2, 8},   // 14: 64 x rows, 2 planes, eight waves
     };
     return s;
 }
@@ -413,6 +465,16 @@ static const std::vector<TiledEntry>& entries() {
         add_shape<32, 4, 4, 2>(v);
         add_shape<32, 2, 8, 4>(v);
         add_shape<64, 4, 4, 4>(v);
+        add_shape<32, 4, 4, 4>(v);
+        add_shape<32, 2, 6, 4>(v);
+        add_shape<32, 2, 4, 4>(v);
+        add_shape<32, 4, 2, 4>(v);
+        add_shape<64, 2, 4, 4>(v);
+        add_shape<32, 2, 2, 8>(v);
+        add_shape<32, 2, 2, 4>(v);
+        add_shape<32, 2, 2, 2>(v);
+        add_shape<32, 2, 4, 8>(v);
+        add_shape<32, 4, 2, 8>(v);
         return v;
     }();
     return e;
@@ -464,33 +526,51 @@ static bool make_geometry(const pct_problem* pb, int shape_id, int group, TileGe
     g->lds_bytes = e->lds_bytes;
     const int by_lds = kLdsPerCu / e->lds_bytes;
     const int by_waves = 32 / s.waves;  // 8 waves per SIMD slot budget, 4 SIMDs
-    g->wg_per_cu = std::max(1, std::min(by_lds, by_waves));
+    g->wg_per_cu = e->occupancy > 0 ? e->occupancy : std::max(1, std::min(by_lds, by_waves));
     return g->n_items < INT_MAX;
 }
 
 // Cost model: issue slots per output-candidate for one lane, including the
 // halo lanes and halo rows, times the number of rounds the grid needs.
+// Relative VALU issue rate a SIMD sustains on this kernel with w resident
+// waves.  Fitted to the tiled kernels measured on MI355X at 222x222x112
+// (n=10, NH=11; n=5, NH=9): one wave per SIMD issues far below the VALU
+// rate, throughput grows with occupancy and saturates around five waves.
+static double simd_efficiency(double w) {
+    if (w <= 1.0) return 0.25 * w;
+    if (w <= 2.0) return 0.25 + 0.30 * (w - 1.0);
+    if (w <= 3.0) return 0.55 + 0.13 * (w - 2.0);
+    if (w <= 4.0) return 0.68 + 0.04 * (w - 3.0);
+    return std::min(0.78, 0.72 + 0.06 * (w - 4.0));
+}
+
+// Cost model: VALU issue slots of one item per wave, times the rounds of
+// items the grid needs, divided by the efficiency the occupancy allows.
+// Per (dx, dy) step the instruction count is the fit
+//   1.7 (BZ+2)(BX+2) + 1.5 (BZ+2) BX + 6.5 BZ BX
+// (squared differences, x sums, then z sums, DPP y sums, exp, accumulation
+// per output), within 5% of the compiled kernels; exp issues as four slots.
 static double model_cost(const pct_problem* pb, int shape_id, int group, TileGeometry* geo) {
     TileGeometry g;
     if (!make_geometry(pb, shape_id, group, &g)) return -1;
     const TileShape& s = shapes()[shape_id];
     const double bx = s.bx, bz = s.bz;
     const double nh = 2 * g.hw + 1;
-    // VALU slots per (dx, dy) step per lane (exp counted as 4 slots)
-    const double step = (bz + 2) * (bx + 2) * 2 + (bz + 2) * bx * 1.5 + bz * bx * (1.5 + 2 + 4 + 2);
-    // staging slots per dz step per lane (loads + stores + scaling)
+    const double step = 1.7 * (bz + 2) * (bx + 2) + 1.5 * (bz + 2) * bx + 6.5 * bz * bx +
+                        3.0 * bz * bx;
+    // staging per dz step: address, load, scale, LDS store per element
     const int S = 64 / s.seg_width;
     const double rows = (2 * bz + 2) * (s.waves * S * s.bx + 2 + 2 * g.hw);
-    const double stage = rows / s.waves * 6.0 * ((s.seg_width + 2 * g.hw + 63) / 64);
-    const double wave_time = g.group * nh * (nh * nh * step + stage);
-    // a CU runs 4 SIMDs; waves of all resident workgroups share them
-    const double waves_per_cu = g.wg_per_cu * s.waves;
-    const double cu_rate = std::min(4.0, waves_per_cu) / waves_per_cu;  // share of a SIMD per wave
-    const double item_time = wave_time / cu_rate / g.wg_per_cu;          // per item, amortised
-    const double slots = (double)kNumCu * g.wg_per_cu;
-    const double rounds = std::ceil((double)g.n_items / slots);
+    const double stage = rows / s.waves * 8.0 * ((s.seg_width + 2 * g.hw + 63) / 64);
+    // one-segment (64-lane) shapes measured about 1.5x slower than their
+    // instruction counts predict
+    const double width_penalty = s.seg_width == 64 ? 1.5 : 1.0;
+    const double wave_time = width_penalty * g.group * nh * (nh * nh * step + stage);
+    const double w = (double)g.wg_per_cu * s.waves / 4.0;  // waves per SIMD
+    const double item_time = wave_time * std::max(1.0, w) / simd_efficiency(w);
+    const double rounds = std::ceil((double)g.n_items / ((double)kNumCu * g.wg_per_cu));
     if (geo) *geo = g;
-    return rounds * item_time * g.wg_per_cu;
+    return rounds * item_time;
 }
 
 struct ShapeKey {
@@ -566,11 +646,19 @@ static int check_device(int* dev_out) {
         st.ok = hipGetDeviceProperties(&prop, dev) == hipSuccess &&
                 std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
         if (st.ok) {
-            for (const auto& e : entries())
-                if (e.lds_bytes <= kLdsPerCu)
-                    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(e.fn),
-                                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              e.lds_bytes);
+            for (const auto& e : entries()) {
+                if (e.lds_bytes > kLdsPerCu) continue;
+                (void)hipFuncSetAttribute(reinterpret_cast<const void*>(e.fn),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          e.lds_bytes);
+                int n = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &n, reinterpret_cast<const void*>(e.fn), e.shape.waves * 64,
+                        (size_t)e.lds_bytes) == hipSuccess && n > 0)
+                    e.occupancy = n;
+                else
+                    (void)hipGetLastError();
+            }
         }
     }
     if (dev_out) *dev_out = dev;

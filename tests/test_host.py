@@ -1,0 +1,194 @@
+"""Host-side code of libpct (no GPU): MatrixIO, the configuration file,
+generators, pre/post-processing, the CLI surface.  The reference's own
+functions (oracle/_ref) are the yardstick where they exist."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import pseudoctimaging_amd as pct
+from pseudoctimaging_amd import _lib
+
+CLI = os.path.join(os.path.dirname(_lib.LIB_PATH), "pct_cli")
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+# ---- MatrixIO ---------------------------------------------------------------
+
+def test_matrixio_write_matches_reference_bytes(ref, tmp_path):
+    rng = np.random.default_rng(0)
+    a = (rng.standard_normal(2 * 3 * 4 * 5) * 1000).astype(np.float32)
+    a[:5] = [0.0, -0.0, 1e-7, 123456.78, -3.5]
+    p1, p2 = str(tmp_path / "ours.txt"), str(tmp_path / "ref.txt")
+    pct.matrixio_write(p1, a, 2, 3, 4, 5)
+    oracle.ref_matrixio_store(p2, a, 2, 3, 4, 5)
+    assert open(p1, "rb").read() == open(p2, "rb").read()
+
+
+def test_matrixio_read_matches_reference(ref, tmp_path):
+    rng = np.random.default_rng(1)
+    a = (rng.random(3 * 4 * 5 * 2) * 255).astype(np.float32)
+    p = str(tmp_path / "m.txt")
+    oracle.ref_matrixio_store(p, a, 2, 3, 4, 5)
+    ours, dims = pct.matrixio_read(p)
+    theirs, rdims = oracle.ref_matrixio_read(p)
+    assert dims == rdims == [3, 4, 5, 2]
+    assert np.array_equal(bits(ours), bits(theirs))
+    # 6 significant digits survive the text round trip
+    np.testing.assert_allclose(ours, a, rtol=1e-5)
+
+
+def test_matrixio_errors(tmp_path):
+    p = str(tmp_path / "bad.txt")
+    with open(p, "w") as f:
+        f.write("1\n2\n2\n2\n1 2 3\n")          # 3 values for a 2x2x2 volume
+    with pytest.raises(pct.PctError) as e:
+        pct.matrixio_read(p)
+    assert e.value.status == _lib.PCT_ERR_FORMAT
+    with pytest.raises(pct.PctError) as e:
+        pct.matrixio_read(str(tmp_path / "missing.txt"))
+    assert e.value.status == _lib.PCT_ERR_IO
+    with open(p, "w") as f:
+        f.write("1\n1\n1\n2\n1 x\n")
+    with pytest.raises(pct.PctError):
+        pct.matrixio_read(p)
+
+
+# ---- configuration file (cpp:908-1018) --------------------------------------
+
+def _write_set(tmp_path, img, atlas, label):
+    n, nz, nx, ny = atlas.shape
+    pa, pl, pi = (str(tmp_path / f) for f in ("atlas.txt", "label.txt", "im.txt"))
+    # MatrixIO dims are num, rows (nx), cols (ny), depth (nz), values in memory order
+    pct.matrixio_write(pa, atlas, n, nx, ny, nz)
+    pct.matrixio_write(pl, label, n, nx, ny, nz)
+    pct.matrixio_write(pi, img, 1, nx, ny, nz)
+    cfg = str(tmp_path / "data.cfg")
+    with open(cfg, "w") as f:
+        f.write(f"0\n{pa}\n{pl}\n{pi}\n")
+    return cfg
+
+
+def test_read_config_text_mode_matches_reference(ref, tmp_path):
+    img, amri, act = pct.synth_tissue(7, 8, 6, 2, 1, False, 3)
+    cfg = _write_set(tmp_path, img, amri, act)
+    ours = pct.read_config(cfg)
+    theirs = oracle.ref_read_config(cfg)
+    for k in ("img", "atlas", "label"):
+        assert ours[k].shape == theirs[k].shape
+        assert np.array_equal(bits(ours[k]), bits(theirs[k]))
+
+
+def test_read_config_dimension_mismatch(tmp_path):
+    img, amri, act = pct.synth_tissue(7, 8, 6, 2, 1, False, 3)
+    cfg = _write_set(tmp_path, img, amri, act)
+    pct.matrixio_write(str(tmp_path / "im.txt"), np.zeros(6 * 7 * 9, np.float32), 1, 7, 9, 6)
+    with pytest.raises(pct.PctError) as e:
+        pct.read_config(cfg)
+    assert e.value.status == _lib.PCT_ERR_DIMS
+
+
+def test_read_config_random_mode(tmp_path):
+    cfg = str(tmp_path / "rand.cfg")
+    with open(cfg, "w") as f:
+        f.write("1\n3\n9\n10\n11\n")   # random flag, num, nx, ny, nz
+    a = pct.read_config(cfg, seed=5)
+    b = pct.read_config(cfg, seed=5)
+    c = pct.read_config(cfg, seed=6)
+    assert a["img"].shape == (11, 9, 10) and a["atlas"].shape == (3, 11, 9, 10)
+    for k in ("img", "atlas", "label"):
+        assert np.array_equal(a[k], b[k])
+        assert not np.array_equal(a[k], c[k])
+        assert a[k].min() >= 0 and a[k].max() < 10
+    assert abs(a["atlas"].mean() - 5) < 0.2
+
+
+# ---- generators ---------------------------------------------------------------
+
+def test_tissue_generator_properties():
+    img, amri, act = pct.synth_tissue(40, 44, 36, 3, 2, False, 11)
+    img2, amri2, act2 = pct.synth_tissue(40, 44, 36, 3, 2, False, 11)
+    assert np.array_equal(bits(img), bits(img2)) and np.array_equal(bits(act), bits(act2))
+    # four classes at the quartiles: CT class means -1000/-100/40/1000, sd 20
+    cls = np.digitize(act[0], [-550, -30, 520])
+    frac = np.bincount(cls.ravel(), minlength=4) / cls.size
+    assert np.all(np.abs(frac - 0.25) < 0.06), frac
+    for c, m in enumerate([-1000, -100, 40, 1000]):
+        v = act[0][cls == c]
+        assert abs(v.mean() - m) < 2 and abs(v.std() - 20) < 2
+    # MRI class means 0/0.8/0.5/0.2 with sd 0.03, same anatomy as the CT
+    for c, m in enumerate([0.0, 0.8, 0.5, 0.2]):
+        v = amri[0][cls == c]
+        assert abs(v.mean() - m) < 0.01 and abs(v.std() - 0.03) < 0.005
+
+
+def test_shared_anatomy_is_a_shift_of_the_target():
+    img, amri, act = pct.synth_tissue(30, 32, 28, 2, 2, False, 4)
+    tcls = np.digitize(img, [0.1, 0.35, 0.65])      # MRI: 0 air, 0.2 bone, 0.5 soft, 0.8 fat
+    acls = np.digitize(amri[0], [0.1, 0.35, 0.65])
+    best = max(((dz, dx, dy) for dz in range(-2, 3) for dx in range(-2, 3) for dy in range(-2, 3)),
+               key=lambda s: np.mean(acls[4:-4, 4:-4, 4:-4] ==
+                                     tcls[4 + s[0]:24 + s[0], 4 + s[1]:26 + s[1],
+                                          4 + s[2]:28 + s[2]]))
+    dz, dx, dy = best
+    agree = np.mean(acls[4:-4, 4:-4, 4:-4] == tcls[4 + dz:24 + dz, 4 + dx:26 + dx, 4 + dy:28 + dy])
+    assert agree > 0.99
+    ind = pct.synth_tissue(30, 32, 28, 2, 2, True, 4)[1]
+    iagree = np.mean(np.digitize(ind[0], [0.1, 0.35, 0.65]) == tcls)
+    assert iagree < 0.6
+
+
+def test_uniform_generator():
+    img, atlas, label = pct.synth_uniform(8, 9, 10, 2, 7)
+    assert img.shape == (10, 8, 9) and atlas.shape == (2, 10, 8, 9)
+    assert img.min() >= 0 and img.max() < 10
+
+
+# ---- pre/post-processing -----------------------------------------------------
+
+def test_normalize_and_divisor_match_oracle():
+    rng = np.random.default_rng(2)
+    v = (rng.random(10000) * 4095 - 1024).astype(np.float32)
+    a, b = v.copy(), v.copy()
+    pct.normalize_labels(a)
+    oracle.normalize_labels(b)
+    assert np.array_equal(bits(a), bits(b))
+    for s, be in [(2.5, 1.0), (1.0, 1.0), (0.3, 2.0)]:
+        assert pct.weight_divisor(s, be) == oracle.div(s, be)
+
+
+@pytest.mark.parametrize("name", ["small_tissue.npz", "small_uniform.npz"])
+def test_postprocess_matches_golden(name):
+    g = dict(np.load(os.path.join(os.path.dirname(__file__), "golden", name)))
+    nh = int(g["params"][4])
+    hu = pct.postprocess(g["fused"].copy(), nh)
+    assert np.array_equal(bits(hu), bits(g["hu"]))
+
+
+def test_regularize_stops_without_progress():
+    f = np.full((8, 8, 8), np.nan, np.float32)
+    passes = pct.regularization_step(f, 3)   # nothing can ever be filled
+    assert passes == 1 and np.isnan(f[2:-2, 2:-2, 2:-2]).all()
+
+
+# ---- CLI surface ----------------------------------------------------------------
+
+def test_cli_usage_without_arguments():
+    r = subprocess.run([CLI], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "EXEC_CHOICE" in r.stdout
+
+
+def test_cli_fails_cleanly_without_device(tmp_path):
+    if pct.device_count() > 0:
+        pytest.skip("a GPU is present")
+    cfg = str(tmp_path / "rand.cfg")
+    with open(cfg, "w") as f:
+        f.write("1\n1\n12\n12\n12\n")
+    r = subprocess.run([CLI, cfg, str(tmp_path / "out"), "2", "3"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode != 0 and "gfx950" in r.stderr
