@@ -66,6 +66,111 @@ def cpu_baseline(img, amri, lab, nh, sigma, beta, planes, rows, threads):
     return kind, dt, vox_sample
 
 
+# The uneven batch: (nx, ny, nz, atlases, NH) per subject
+BATCH = [(160, 160, 96, 1, 7), (192, 192, 96, 3, 9), (222, 222, 112, 5, 11),
+         (222, 222, 112, 7, 11), (256, 256, 128, 10, 9), (128, 128, 64, 10, 7)]
+
+
+def batched_leg(steps: int, warmup: int, seed: int):
+    """Six subjects of uneven size, atlas count and window in one
+    graph-captured persistent launch; returns a result dict."""
+    import numpy as np
+    import torch
+
+    import pseudoctimaging_amd as pct
+    dev = torch.device("cuda:0")
+    subs, vox, cvp = [], 0, 0
+    for k, (nx, ny, nz, n, nh) in enumerate(BATCH):
+        img, amri, act = pct.synth_tissue(nx, ny, nz, n, 3, False, seed + 100 + k)
+        lab = act.copy()
+        pct.normalize_labels(lab)
+        subs.append({"img": torch.from_numpy(img).to(dev), "atlas": torch.from_numpy(amri).to(dev),
+                     "label": torch.from_numpy(lab).to(dev),
+                     "out": torch.empty(img.shape, device=dev), "nh": nh, "sigma": 2.5,
+                     "beta": 1.0})
+        r = (nh - 1) // 2 + 1
+        vox += nx * ny * nz
+        cvp += interior(nx, r) * interior(ny, r) * interior(nz, r) * n * nh ** 3
+        del img, amri, act, lab
+    tuned = pct.autotune_batch(subs, reps=1)
+    for _ in range(warmup):
+        pct.fuse_batch(subs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pct.fuse_batch(subs)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    finite = all(bool(torch.isfinite(s["out"]).all()) for s in subs)
+    return {"subjects": [list(b) for b in BATCH], "ms_per_batch": dt * 1e3,
+            "voxels_per_s": vox / dt, "cvp_per_s": cvp / dt,
+            "achieved_tflops_86_per_cvp": FLOPS_PER_CVP * cvp / dt / 1e12,
+            "config": tuned, "outputs_finite": finite, "launch": "one hipGraph per batch"}
+
+
+def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup, seed,
+                plain_ms, train_shape=(64, 96, 96), train_subjects=4, train_rounds=2,
+                train_steps=60, topk=32, samples=27, margin=0.0):
+    """Learned-pruned fusion on the resident inputs: the embedding MLP is
+    trained on the GPU on `train_subjects` other seeded phantoms, then the
+    pruned fusion is timed and compared with the plain output."""
+    import numpy as np
+    import torch
+
+    import pseudoctimaging_amd as pct
+    from pseudoctimaging_amd import learned
+    dev = d_img.device
+    n = d_amri.shape[0]
+    tz, tx, ty = train_shape
+    subjects = []
+    for k in range(train_subjects):
+        img, amri, _ = pct.synth_tissue(tx, ty, tz, n, 3, False, seed + 1000 + k)
+        subjects.append((torch.from_numpy(img).to(dev), torch.from_numpy(amri).to(dev)))
+    mu, inv_sd = learned.intensity_stats(d_img.cpu().numpy())
+    params = learned.mlp_init(seed, mu, inv_sd)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = []
+    for rnd in range(train_rounds):
+        for k, (ti, ta) in enumerate(subjects):
+            params, trace = learned.mlp_train(ti, ta, nh, sigma, beta, params, steps=train_steps,
+                                              anchors=256, lr=3e-3, seed=seed + 10 * rnd + k)
+            losses.append([float(trace[0]), float(trace[-1])])
+    train_s = time.perf_counter() - t0
+    del subjects
+    out = torch.empty_like(plain_out)
+    frac = learned.fuse_pruned_device(d_img, d_amri, d_lab, out, nh, sigma, beta, params,
+                                      topk=topk, samples=samples, margin=margin)
+    for _ in range(warmup):
+        learned.fuse_pruned_device(d_img, d_amri, d_lab, out, nh, sigma, beta, params, topk=topk,
+                                   samples=samples, margin=margin, report_fallback=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        learned.fuse_pruned_device(d_img, d_amri, d_lab, out, nh, sigma, beta, params, topk=topk,
+                                   samples=samples, margin=margin, report_fallback=False)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    a = out.cpu().numpy().astype(np.float64)
+    b = plain_out.cpu().numpy().astype(np.float64)
+    ok = np.isfinite(a) & np.isfinite(b)
+    dhu = np.abs(4095.0 * (a[ok] - b[ok]) / 255.0)
+    full = int(np.prod(plain_out.shape))
+    return {"ms_per_step": ms, "voxels_per_s": full / (ms * 1e-3),
+            "speedup_vs_plain": plain_ms / ms, "mae_hu_vs_plain": float(dhu.mean()),
+            "max_err_hu_vs_plain": float(dhu.max()), "p99_err_hu_vs_plain":
+            float(np.percentile(dhu, 99)),
+            "nan_mismatch": int(np.count_nonzero(np.isnan(a) != np.isnan(b))),
+            "fallback_frac": frac, "topk": topk, "lattice_samples": samples, "margin": margin,
+            "embedding": "30-64-64-16 MLP, BF16 MFMA inference, FP32 training on GPU",
+            "train": {"subjects": train_subjects, "shape_zxy": list(train_shape),
+                      "rounds": train_rounds, "steps_per_subject": train_steps,
+                      "anchors_per_step": 256, "seconds": train_s,
+                      "loss_first_last": losses},
+            "timed": "embeddings of target and atlases + selection + pruned weighting "
+                     "+ fallback + finalize, per call"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +187,8 @@ def main() -> int:
     ap.add_argument("--cpu-planes", type=int, default=16)
     ap.add_argument("--cpu-rows", type=int, default=48)
     ap.add_argument("--parity-voxels", type=int, default=2000)
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-pruned leg")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -186,6 +293,24 @@ def main() -> int:
                   "nan_mismatch": int(np.count_nonzero(np.isnan(got) != np.isnan(ref)))}
         log(f"parity sample: {parity}")
 
+    learned_res = None
+    if not args.no_learned:
+        log("learned leg: training the embedding on 4 other phantoms, then pruned fusion")
+        learned_res = learned_leg(d_img, d_amri, d_lab, d_out, nh, sigma, beta,
+                                  max(2, args.steps // 2), 1, args.seed, ms_per_step)
+        log(f"learned: {learned_res['ms_per_step']:.2f} ms/step, speedup "
+            f"{learned_res['speedup_vs_plain']:.2f}x, MAE {learned_res['mae_hu_vs_plain']:.3f} HU, "
+            f"fallback {learned_res['fallback_frac']:.3f}")
+
+    batched = None
+    if not args.no_batch:
+        del d_img, d_amri, d_lab, d_out
+        torch.cuda.empty_cache()
+        log("batched leg: six uneven subjects in one graph launch")
+        batched = batched_leg(max(2, args.steps // 2), 1, args.seed)
+        log(f"batched: {batched['ms_per_batch']:.2f} ms per batch, "
+            f"{batched['voxels_per_s']:.4e} voxels/s")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -220,6 +345,8 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity_sample": parity,
+        "learned_pruned": learned_res,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
     return 0

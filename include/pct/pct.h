@@ -142,6 +142,126 @@ int pct_autotune(const pct_problem* pb, const float* d_img, const float* d_atlas
 /* Force a configuration (by id, atlas_group) for a problem shape; -1 clears. */
 int pct_tile_config_force(int id, int atlas_group);
 
+/* ---- batched fusion (no counterpart in the reference, whose wrapper fuses
+ * one subject per call, cu:1635; for callers with many subjects) ---- */
+
+#define PCT_MAX_BATCH 16
+
+/* One subject of a batch: its problem and its device buffers. */
+typedef struct pct_subject {
+    pct_problem pb;
+    const float* d_img;
+    const float* d_atlas;
+    const float* d_atlas_label_norm;
+    float* d_out;
+} pct_subject;
+
+/* Fuses up to PCT_MAX_BATCH subjects of any sizes, atlas counts and windows
+ * in one persistent kernel fed by a work queue (plus the finalize passes),
+ * captured once into a hipGraph and replayed while the descriptors stay the
+ * same.  Each subject's output equals its individual pct_fuse_dev(SM) run
+ * with the same tile configuration bit for bit.  Asynchronous on `stream`;
+ * uses the library's per-device workspace. */
+int pct_fuse_batch_dev(const pct_subject* subjects, int n_subjects, pct_stream_t stream);
+
+/* Bytes of workspace pct_fuse_batch_dev_ws needs. */
+size_t pct_fuse_batch_workspace_size(const pct_subject* subjects, int n_subjects);
+
+/* pct_fuse_batch_dev with a caller-provided workspace (256-byte aligned). */
+int pct_fuse_batch_dev_ws(const pct_subject* subjects, int n_subjects, void* d_workspace,
+                          size_t workspace_bytes, pct_stream_t stream);
+
+/* Measures every batched tile shape and atlas group on these subjects,
+ * keeps the fastest for this set of problem shapes.  Synchronises. */
+int pct_autotune_batch(const pct_subject* subjects, int n_subjects, int reps,
+                       pct_stream_t stream, pct_tile_config* best);
+
+/* Sets the tile shape (pct_tile_config.id) and atlas group used for this set
+ * of problem shapes. */
+int pct_fuse_batch_config(const pct_subject* subjects, int n_subjects, int shape_id,
+                          int atlas_group);
+
+/* ---- learned candidate proposal (no counterpart in the reference, which
+ * always searches all n*NH^3 candidates, cpp:1399-1432; for callers that
+ * trade a bounded HU error for speed) ----
+ *
+ * A per-voxel MLP 30 -> 64 -> 64 -> 16 (ReLU after the first two layers)
+ * embeds target and atlas voxels; candidates whose embeddings are close to
+ * the target's are kept.  Parameter vector, float32, PCT_MLP_NPARAMS values:
+ *   W1[32][64] (input-major; input rows 30, 31 unused), b1[64],
+ *   W2[64][64], b2[64], W3[64][16], b3[16], mu, inv_sd.
+ * Features of voxel u of volume V, with g(u) = (V(u) - mu) * inv_sd inside
+ * the volume and 0 outside:
+ *   f[9*pz+3*px+py] = g(u + (pz-1, px-1, py-1)), then
+ *   f[27..29] = (g(u+e) - g(u-e)) / 2 for e = z, x, y.
+ * Inference runs in BF16 (weights and activations) with FP32 accumulation
+ * on MFMA; training keeps FP32 master weights. */
+#define PCT_MLP_NPARAMS 7314
+
+/* He-initialised parameters (host), seeded; mu / inv_sd as given. */
+int pct_mlp_init(float* params, uint64_t seed, float mu, float inv_sd);
+
+/* 16-dim BF16 embedding (raw 16-bit words, [voxel][16]) of every voxel of
+ * a device volume; `params` is a host parameter vector. */
+int pct_embed_dev(const float* params, const float* d_vol, int nx, int ny, int nz,
+                  uint16_t* d_emb, pct_stream_t stream);
+
+/* Options of the pruned fusion. */
+typedef struct pct_pruned_opts {
+    int topk;        /* candidate offsets kept per tile and atlas (32) */
+    int samples;     /* offsets on a fixed lattice weighted exactly to estimate
+                        the mean weight of the candidates that are not kept */
+    float margin;    /* tiles whose gap between the topk-th and the next
+                        embedding score is below margin times the score
+                        spread (median - best) take the full search; 0 = none,
+                        negative = every tile (validation) */
+    int shape_id;    /* tile shape (pct_tile_config.id); -1 = default */
+} pct_pruned_opts;
+
+/* Learned-pruned fusion.  Per tile of target voxels and per atlas, the
+ * topk candidate offsets with the smallest embedding distance (summed over a
+ * compact probe cluster at the tile centre) are weighted exactly in FP32 for
+ * every voxel of the tile; the remaining candidates enter through their CT
+ * window sum times the mean exact weight of `samples` lattice offsets.  Tiles
+ * with a low selection margin run the full exact search.  Output as
+ * pct_fuse_dev.  If fallback_frac is not NULL the call synchronises and
+ * stores the fraction of (tile, atlas) items that fell back. */
+int pct_fuse_pruned_dev(const pct_problem* pb, const float* params, const pct_pruned_opts* opts,
+                        const float* d_img, const float* d_atlas,
+                        const float* d_atlas_label_norm, float* d_out, double* fallback_frac,
+                        pct_stream_t stream);
+
+/* Training objective.  An anchor is an interior target voxel u and an atlas
+ * a.  Over the NH^3 window offsets c, the teacher distribution is
+ * p_c = softmax_c(-SSD_c / T), SSD_c the SSD of the 3x3x3 MRI patches of the
+ * target at u and of atlas a at u + c: the fusion weights' ordering at a
+ * temperature T (T = div of pct_weight_divisor gives the normalised fusion
+ * weights themselves, nearly uniform for large sigma).  The student
+ * distribution is q_c = softmax_c(-d_c / tau), d_c = |e_T(u) - e_a(u + c)|^2
+ * the squared distance of the FP32 embeddings.  Loss = mean over anchors of
+ * KL(p || q).  mu and inv_sd are not trained. */
+typedef struct pct_train_opts {
+    int steps;       /* Adam steps */
+    int anchors;     /* anchors per step, drawn uniformly (interior, atlas) */
+    float lr;        /* Adam learning rate (beta1 0.9, beta2 0.999, eps 1e-8) */
+    float tau;       /* student temperature */
+    float teacher_t; /* teacher temperature T; <= 0: div */
+    uint64_t seed;   /* anchor sampling */
+} pct_train_opts;
+
+/* Loss and gradient (host, PCT_MLP_NPARAMS floats; 0 for mu, inv_sd and the
+ * unused W1 rows) for the given anchors (host, [n_anchors][4] = z, x, y,
+ * atlas).  Deterministic.  Synchronises. */
+int pct_mlp_loss_grad_dev(const pct_problem* pb, const float* params, const int* anchors,
+                          int n_anchors, float tau, float teacher_t, const float* d_img,
+                          const float* d_atlas, double* loss, float* grad, pct_stream_t stream);
+
+/* Trains `params` (host, updated in place) on the GPU; loss_trace (host,
+ * `steps` values, may be NULL) receives each step's loss.  Synchronises. */
+int pct_mlp_train_dev(const pct_problem* pb, float* params, const pct_train_opts* opts,
+                      const float* d_img, const float* d_atlas, float* loss_trace,
+                      pct_stream_t stream);
+
 /* ---- kernel timing (no counterpart in the reference, which times the whole
  * wrapper with cudaEvents, cu:2152-2188) ---- */
 

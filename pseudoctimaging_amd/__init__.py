@@ -4,6 +4,7 @@ The fusion runs in hand-written HIP kernels inside libpct.so (C ABI in
 include/pct/pct.h); this package is the Python host interface over it.
 """
 from .api import (CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, autotune,  # noqa: F401
+                  autotune_batch, fuse_batch, set_batch_config,
                   device_count, force_tile_config, fuse_device, fused_to_hu_raw,
                   matrixio_read, matrixio_write, multi_patch_based_segmentation_gpu,
                   normalize_labels, normalize_labels_device, postprocess, problem, pseudo_ct,
@@ -11,7 +12,8 @@ from .api import (CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, autotune,  # noqa:
                   tile_config, to_hu, weight_divisor)
 
 __all__ = [
-    "CHOICE_GM", "CHOICE_GM2", "CHOICE_SM", "PctError", "autotune", "device_count",
+    "CHOICE_GM", "CHOICE_GM2", "CHOICE_SM", "PctError", "autotune", "autotune_batch",
+    "fuse_batch", "set_batch_config", "device_count",
     "force_tile_config", "fuse_device", "fused_to_hu_raw", "matrixio_read", "matrixio_write",
     "multi_patch_based_segmentation_gpu", "normalize_labels", "normalize_labels_device",
     "postprocess", "problem", "pseudo_ct", "read_config", "regularization_step", "round_matrix",

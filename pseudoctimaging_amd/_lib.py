@@ -47,12 +47,34 @@ class TileConfig(Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Subject(Structure):
+    """pct_subject"""
+    _fields_ = [("pb", Problem), ("d_img", c_void_p), ("d_atlas", c_void_p),
+                ("d_atlas_label_norm", c_void_p), ("d_out", c_void_p)]
+
+
+PCT_MAX_BATCH = 16
+
+
 class VolumeSet(Structure):
     """pct_volume_set"""
     _fields_ = [("nx", c_int), ("ny", c_int), ("nz", c_int), ("n_atlas", c_int),
                 ("img", POINTER(c_float)), ("atlas", POINTER(c_float)),
                 ("label", POINTER(c_float))]
 
+
+class PrunedOpts(Structure):
+    """pct_pruned_opts"""
+    _fields_ = [("topk", c_int), ("samples", c_int), ("margin", c_float), ("shape_id", c_int)]
+
+
+class TrainOpts(Structure):
+    """pct_train_opts"""
+    _fields_ = [("steps", c_int), ("anchors", c_int), ("lr", c_float), ("tau", c_float),
+                ("teacher_t", c_float), ("seed", c_uint64)]
+
+
+PCT_MLP_NPARAMS = 7314
 
 FP = POINTER(c_float)
 
@@ -74,6 +96,21 @@ SIGNATURES = {
     "pct_autotune": (c_int, [POINTER(Problem), c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                              c_void_p, POINTER(TileConfig)]),
     "pct_tile_config_force": (c_int, [c_int, c_int]),
+    "pct_fuse_batch_dev": (c_int, [POINTER(Subject), c_int, c_void_p]),
+    "pct_fuse_batch_workspace_size": (c_size_t, [POINTER(Subject), c_int]),
+    "pct_fuse_batch_dev_ws": (c_int, [POINTER(Subject), c_int, c_void_p, c_size_t, c_void_p]),
+    "pct_autotune_batch": (c_int, [POINTER(Subject), c_int, c_int, c_void_p,
+                                   POINTER(TileConfig)]),
+    "pct_fuse_batch_config": (c_int, [POINTER(Subject), c_int, c_int, c_int]),
+    "pct_mlp_init": (c_int, [FP, c_uint64, c_float, c_float]),
+    "pct_embed_dev": (c_int, [FP, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "pct_fuse_pruned_dev": (c_int, [POINTER(Problem), FP, POINTER(PrunedOpts), c_void_p, c_void_p,
+                                    c_void_p, c_void_p, POINTER(c_double), c_void_p]),
+    "pct_mlp_loss_grad_dev": (c_int, [POINTER(Problem), FP, POINTER(c_int), c_int, c_float,
+                                      c_float, c_void_p, c_void_p, POINTER(c_double), FP,
+                                      c_void_p]),
+    "pct_mlp_train_dev": (c_int, [POINTER(Problem), FP, POINTER(TrainOpts), c_void_p, c_void_p,
+                                  FP, c_void_p]),
     "pct_kernel_timing": (c_int, [c_int]),
     "pct_kernel_timing_read": (c_int, [POINTER(c_double), POINTER(c_int)]),
     "pct_regularize": (c_int, [c_int, c_int, c_int, c_int, FP]),
@@ -111,6 +148,13 @@ def lib() -> ctypes.CDLL:
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with pseudoctimaging_amd.build.build() "
                 "(the fusion path has no CPU fallback)")
+        # torch ships its own libamdhip64.so.7.  Loaded after libpct.so, it
+        # would bind to the runtime libpct.so brought in (same soname) and
+        # fail to initialise; loaded first, both share torch's runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         l = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(l, name)

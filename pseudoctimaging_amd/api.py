@@ -142,6 +142,51 @@ def force_tile_config(config_id: int, atlas_group: int = 0) -> None:
     check(lib().pct_tile_config_force(int(config_id), int(atlas_group)), "pct_tile_config_force")
 
 
+# ---- batches of subjects -------------------------------------------------------
+
+def _subjects(subjects):
+    """ctypes array of pct_subject from dicts with device tensors img, atlas,
+    label (normalised CT), out and the window parameters nh, sigma, beta."""
+    arr = (_lib.Subject * len(subjects))()
+    for i, s in enumerate(subjects):
+        arr[i].pb = problem(s["img"].shape, s["atlas"].shape[0], s["nh"], s.get("sigma", 1.0),
+                            s.get("beta", 1.0))
+        arr[i].d_img = _dev_ptr(s["img"], "img")
+        arr[i].d_atlas = _dev_ptr(s["atlas"], "atlas")
+        arr[i].d_atlas_label_norm = _dev_ptr(s["label"], "label")
+        arr[i].d_out = _dev_ptr(s["out"], "out")
+    return arr
+
+
+def fuse_batch(subjects, stream=None) -> None:
+    """Fuses every subject (list of dicts, see _subjects) in one graph-captured
+    persistent launch; asynchronous on `stream`."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    arr = _subjects(subjects)
+    check(lib().pct_fuse_batch_dev(arr, len(subjects), ctypes.c_void_p(stream.cuda_stream)),
+          "pct_fuse_batch_dev")
+
+
+def autotune_batch(subjects, reps: int = 1, stream=None) -> dict:
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    arr = _subjects(subjects)
+    cfg = TileConfig()
+    check(lib().pct_autotune_batch(arr, len(subjects), int(reps),
+                                   ctypes.c_void_p(stream.cuda_stream), byref(cfg)),
+          "pct_autotune_batch")
+    return cfg.as_dict()
+
+
+def set_batch_config(subjects, config_id: int, atlas_group: int) -> None:
+    arr = _subjects(subjects)
+    check(lib().pct_fuse_batch_config(arr, len(subjects), int(config_id), int(atlas_group)),
+          "pct_fuse_batch_config")
+
+
 # ---- post-processing --------------------------------------------------------
 
 def regularization_step(arr: np.ndarray, nh: int) -> int:

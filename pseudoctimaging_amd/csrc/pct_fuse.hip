@@ -38,292 +38,9 @@
 
 #include "pct_internal.h"
 
+#include "pct_tiled.h"
+
 namespace pct {
-
-// ---------------------------------------------------------------------------
-// device helpers
-// ---------------------------------------------------------------------------
-
-// lane l receives lane l-1 (lane 0 receives 0).
-__device__ __forceinline__ float lane_prev(float v) {
-    return __builtin_bit_cast(
-        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
-}
-// lane l receives lane l+1 (lane 63 receives 0).
-__device__ __forceinline__ float lane_next(float v) {
-    return __builtin_bit_cast(
-        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
-}
-
-// Work items are dispatched round-robin over the 8 XCDs; give each XCD a
-// contiguous run of items so that tiles that share halos share an L2.
-__device__ __forceinline__ int xcd_contiguous(int b, int n) {
-    const int per = n >> 3, rem = n & 7;
-    const int xcd = b & 7, local = b >> 3;
-    return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
-}
-
-// The reference's rule (cpp:1437-1443) in float: DBL_EPSILON = 2^-52 is exact.
-__device__ __forceinline__ float fused_value(float P, float S) {
-    const float eps = 2.220446049250313080847e-16f;
-    return (fabsf(P) > eps && fabsf(S) > eps) ? (P / S) : __builtin_nanf("");
-}
-
-// ---------------------------------------------------------------------------
-// tiled kernel
-// ---------------------------------------------------------------------------
-
-struct TiledArgs {
-    const float* img;
-    const float* atlas;
-    const float* label;
-    float* ws;        // [n_groups][2][N]
-    long long N;      // nx*ny*nz
-    int nx, ny, nz;
-    int r;            // hw + 1
-    int tiles_y, tiles_x;
-    int n_tiles;
-    int n_items;
-    int n_atlas, group;
-    float scale;      // sqrt(log2(e)/div)
-};
-
-template <int W, int BX, int BZ, int NW, int HW>
-struct TiledCfg {
-    static constexpr int S = 64 / W;           // segments per wave
-    static constexpr int NSEG = NW * S;        // segments per workgroup (stacked in x)
-    static constexpr int WX = NSEG * BX;       // output rows per workgroup
-    static constexpr int OY = W - 2;           // outputs per segment in y
-    static constexpr int CW = W + 2 * HW;      // LDS row length (y)
-    static constexpr int RX = WX + 2 + 2 * HW; // LDS rows per plane (x)
-    static constexpr int PS = RX * CW;         // LDS plane stride
-    static constexpr int APLANES = BZ + 2;     // atlas MRI planes per stage
-    static constexpr int LPLANES = BZ;         // atlas CT planes per stage
-    static constexpr int LOFF = APLANES * PS;  // CT region offset
-    static constexpr int LDS_FLOATS = (APLANES + LPLANES) * PS;
-    static constexpr int LDS_BYTES = LDS_FLOATS * 4;
-    static constexpr int COLCHUNKS = (CW + 63) / 64;
-    // ds_read_b32 takes a 16-bit byte offset: planes are addressed from
-    // several base registers, PPB planes (plus row offsets) per base.
-    static constexpr int PPB = (16383 - (BX + 1) * CW) / PS + 1;
-    static_assert(PPB >= 1, "one plane must fit an LDS immediate window");
-    static_assert(64 % W == 0, "segment width must divide the wavefront");
-    static_assert(BX % 2 == 0 && BZ % 2 == 0, "pairwise box sums need even blocks");
-};
-
-// Hides an LDS element index from the optimiser so that it stays one base
-// register instead of being folded into offsets that do not fit an LDS
-// immediate.
-__device__ __forceinline__ int opaque(int v) {
-    asm volatile("" : "+v"(v));
-    return v;
-}
-
-// One (dx, dy) candidate step for the lane's BZ x BX block.  `base` is the
-// LDS element index of plane 0, row 0 of the lane's window for this step.
-template <class C, int BX, int BZ>
-__device__ __forceinline__ void tiled_step(const float* __restrict__ lds, int base,
-                                           const float (&tv)[BZ + 2][BX + 2],
-                                           float (&P)[BZ][BX], float (&S)[BZ][BX]) {
-    constexpr int NBA = (BZ + 2 + C::PPB - 1) / C::PPB;   // bases for the MRI planes
-    constexpr int NBL = (BZ + C::PPB - 1) / C::PPB;       // bases for the CT planes
-    int ba[NBA];
-    int bl[NBL];
-#pragma unroll
-    for (int b = 0; b < NBA; ++b) ba[b] = b ? opaque(base + b * C::PPB * C::PS) : base;
-#pragma unroll
-    for (int b = 0; b < NBL; ++b) bl[b] = opaque(base + C::LOFF + b * C::PPB * C::PS);
-
-    float ex[BZ + 2][BX];
-#pragma unroll
-    for (int pz = 0; pz < BZ + 2; ++pz) {
-        float d[BX + 2];
-        const int pp = ba[pz / C::PPB] + (pz % C::PPB) * C::PS;
-#pragma unroll
-        for (int i = 0; i < BX + 2; ++i) {
-            const float df = tv[pz][i] - lds[pp + i * C::CW];
-            d[i] = df * df;
-        }
-#pragma unroll
-        for (int i = 0; i < BX; i += 2) {
-            const float q = d[i + 1] + d[i + 2];
-            ex[pz][i] = d[i] + q;
-            ex[pz][i + 1] = q + d[i + 3];
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < BZ; j += 2) {
-#pragma unroll
-        for (int i = 0; i < BX; ++i) {
-            const float q = ex[j + 1][i] + ex[j + 2][i];
-            const float e0 = ex[j][i] + q;
-            const float e1 = q + ex[j + 3][i];
-            const float s0 = lane_next(e0) + (lane_prev(e0) + e0);
-            const float s1 = lane_next(e1) + (lane_prev(e1) + e1);
-            const float w0 = __builtin_amdgcn_exp2f(-s0);
-            const float w1 = __builtin_amdgcn_exp2f(-s1);
-            const float l0 = lds[bl[j / C::PPB] + (j % C::PPB) * C::PS + (i + 1) * C::CW];
-            const float l1 =
-                lds[bl[(j + 1) / C::PPB] + ((j + 1) % C::PPB) * C::PS + (i + 1) * C::CW];
-            P[j][i] = fmaf(w0, l0, P[j][i]);
-            S[j][i] += w0;
-            P[j + 1][i] = fmaf(w1, l1, P[j + 1][i]);
-            S[j + 1][i] += w1;
-        }
-    }
-}
-
-// Copies the slab of one (atlas, dz) stage into LDS: plane p < APLANES holds
-// the atlas MRI at z = zdz-1+p (scaled), plane APLANES+j the atlas CT at
-// z = zdz+j; row rr is volume x = xs+rr, column c is volume y = ys+c; zero
-// outside the volume.  Each wave issues the loads of all its rows of two
-// planes before it writes any of them, so global latency is paid once per
-// pair of planes.
-template <class C, int NW>
-__device__ __forceinline__ void stage_slab(float* __restrict__ lds, const float* __restrict__ A,
-                                           const float* __restrict__ L, float scale, int zdz,
-                                           int xs, int ys, int nx, int ny, int nz,
-                                           long long plane, int wv, int lane) {
-    constexpr int NP = C::APLANES + C::LPLANES;
-    constexpr int RPW = (C::RX + NW - 1) / NW;   // rows per wave and plane
-    constexpr int PB = 2;                         // planes per batch
-    static_assert(NP % PB == 0, "plane batches must tile the slab");
-#pragma unroll 1
-    for (int p0 = 0; p0 < NP; p0 += PB) {
-        float v[PB][RPW][C::COLCHUNKS];
-#pragma unroll
-        for (int q = 0; q < PB; ++q) {
-            const int p = p0 + q;
-            const bool is_mri = p < C::APLANES;
-            const int zz = is_mri ? zdz - 1 + p : zdz + (p - C::APLANES);
-            const float* __restrict__ src = is_mri ? A : L;
-            const float mul = is_mri ? scale : 1.f;
-            const bool zok = zz >= 0 && zz < nz;
-            const long long zoff = (long long)zz * plane;
-#pragma unroll
-            for (int k = 0; k < RPW; ++k) {
-                const int rr = wv + k * NW;
-                const int xx = xs + rr;
-                const bool xok = zok && rr < C::RX && xx >= 0 && xx < nx;
-#pragma unroll
-                for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
-                    const int c = lane + cc * 64;
-                    const int yy = ys + c;
-                    float t = 0.f;
-                    if (xok && c < C::CW && yy >= 0 && yy < ny)
-                        t = src[zoff + (long long)xx * ny + yy] * mul;
-                    v[q][k][cc] = t;
-                }
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < PB; ++q)
-#pragma unroll
-            for (int k = 0; k < RPW; ++k) {
-                const int rr = wv + k * NW;
-#pragma unroll
-                for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
-                    const int c = lane + cc * 64;
-                    if (rr < C::RX && c < C::CW) lds[(p0 + q) * C::PS + rr * C::CW + c] = v[q][k][cc];
-                }
-            }
-    }
-}
-
-// One work item (tile, atlas group) of one problem: the whole workgroup runs
-// it.  Shared by the single-problem kernel and the persistent batched kernel,
-// so a problem gives bit-identical results through either.
-template <int W, int BX, int BZ, int NW, int HW>
-__device__ __forceinline__ void tiled_item(const TiledArgs& args, int item,
-                                           float* __restrict__ lds) {
-    using C = TiledCfg<W, BX, BZ, NW, HW>;
-    const int g = item / args.n_tiles;
-    int t = item - g * args.n_tiles;
-    const int ty = t % args.tiles_y;
-    t /= args.tiles_y;
-    const int tx = t % args.tiles_x;
-    const int tz = t / args.tiles_x;
-
-    const int tid = threadIdx.x;
-    const int wv = tid >> 6;
-    const int lane = tid & 63;
-    const int seg = lane / W;
-    const int ly = lane % W;
-    const int sg = wv * C::S + seg;
-
-    const int nx = args.nx, ny = args.ny, nz = args.nz, r = args.r;
-    const long long plane = (long long)nx * ny;
-    const int y0 = r + ty * C::OY;
-    const int x0 = r + tx * C::WX;
-    const int z0 = r + tz * BZ;
-    const int y = y0 - 1 + ly;        // >= 0 because r >= 1
-    const int xb = x0 + sg * BX;      // first output row of this lane
-
-    // Scaled target values of the lane's block plus a one-voxel halo in z
-    // and x (y halo comes from the neighbouring lanes).  Zero outside the
-    // volume: such values only reach outputs that are never stored.
-    float tv[BZ + 2][BX + 2];
-#pragma unroll
-    for (int pz = 0; pz < BZ + 2; ++pz)
-#pragma unroll
-        for (int i = 0; i < BX + 2; ++i) {
-            const int zz = z0 - 1 + pz, xx = xb - 1 + i;
-            const bool ok = zz < nz && xx < nx && y < ny;
-            tv[pz][i] = ok ? args.img[zz * plane + (long long)xx * ny + y] * args.scale : 0.f;
-        }
-
-    float P[BZ][BX], S[BZ][BX];
-#pragma unroll
-    for (int j = 0; j < BZ; ++j)
-#pragma unroll
-        for (int i = 0; i < BX; ++i) P[j][i] = S[j][i] = 0.f;
-
-    const int lane_base = sg * BX * C::CW + ly;
-    const int xs = x0 - 1 - HW;   // volume x of LDS row 0
-    const int ys = y0 - 1 - HW;   // volume y of LDS column 0
-    const int a_begin = g * args.group;
-    const int a_end = min(args.n_atlas, a_begin + args.group);
-
-    for (int a = a_begin; a < a_end; ++a) {
-        const float* __restrict__ A = args.atlas + (long long)a * args.N;
-        const float* __restrict__ L = args.label + (long long)a * args.N;
-        for (int dz = -HW; dz <= HW; ++dz) {
-            __syncthreads();
-            stage_slab<C, NW>(lds, A, L, args.scale, z0 + dz, xs, ys, nx, ny, nz, plane, wv, lane);
-            __syncthreads();
-#pragma unroll 1
-            for (int dx = 0; dx <= 2 * HW; ++dx) {
-#pragma unroll 1
-                for (int dy = 0; dy <= 2 * HW; ++dy) {
-                    tiled_step<C, BX, BZ>(lds, lane_base + dx * C::CW + dy, tv, P, S);
-                }
-            }
-        }
-    }
-
-    // Store the group's partial sums for the lane's valid outputs.
-    if (ly >= 1 && ly <= W - 2 && y < ny - r) {
-        float* __restrict__ wsP = args.ws + (long long)(2 * g) * args.N;
-        float* __restrict__ wsS = wsP + args.N;
-#pragma unroll
-        for (int j = 0; j < BZ; ++j)
-#pragma unroll
-            for (int i = 0; i < BX; ++i) {
-                const int z = z0 + j, x = xb + i;
-                if (z < nz - r && x < nx - r) {
-                    const long long idx = z * plane + (long long)x * ny + y;
-                    wsP[idx] = P[j][i];
-                    wsS[idx] = S[j][i];
-                }
-            }
-    }
-}
-
-template <int W, int BX, int BZ, int NW, int HW>
-__global__ __launch_bounds__(NW * 64) void fuse_tiled(TiledArgs args) {
-    extern __shared__ float lds[];
-    tiled_item<W, BX, BZ, NW, HW>(args, xcd_contiguous(blockIdx.x, args.n_items), lds);
-}
 
 // Sum of the per-group partials in group order; border voxels get 0
 // (the reference zeroes its output, cu:1693 / cpp:1606).
@@ -436,7 +153,7 @@ void add_shape(std::vector<TiledEntry>& v) {
 }
 
 // The shapes compiled into the library (index = pct_tile_config.id).
-static const std::vector<TileShape>& shapes() {
+const std::vector<TileShape>& shapes() {
     static const std::vector<TileShape> s = {
         {64, 4, 8, 4},   // 0: one 62-wide y segment per wave, 16 x rows per workgroup
         {32, 4, 8, 4},   // 1: two 30-wide y segments per wave, 32 x rows
@@ -487,7 +204,6 @@ static const TiledEntry* find_entry(int shape_id, int hw) {
     return &e;
 }
 
-constexpr int kLdsPerCu = 160 * 1024;
 constexpr int kNumCu = 256;
 
 int validate_problem(const pct_problem* pb) {
@@ -504,7 +220,7 @@ int validate_problem(const pct_problem* pb) {
 
 static int interior(int n, int r) { return std::max(0, n - 2 * r); }
 
-static bool make_geometry(const pct_problem* pb, int shape_id, int group, TileGeometry* g) {
+bool make_geometry(const pct_problem* pb, int shape_id, int group, TileGeometry* g) {
     const int hw = (pb->nh - 1) / 2;
     const TiledEntry* e = find_entry(shape_id, hw);
     if (!e) return false;
@@ -635,7 +351,7 @@ struct DeviceState {
 static std::mutex g_dev_mutex;
 static std::map<int, DeviceState> g_dev;
 
-static int check_device(int* dev_out) {
+int check_device(int* dev_out) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return PCT_ERR_NO_DEVICE;
     std::lock_guard<std::mutex> lk(g_dev_mutex);
@@ -665,7 +381,7 @@ static int check_device(int* dev_out) {
     return st.ok ? PCT_OK : PCT_ERR_NO_DEVICE;
 }
 
-static int ensure_workspace(int dev, size_t bytes, void** ptr) {
+int ensure_workspace(int dev, size_t bytes, void** ptr) {
     std::lock_guard<std::mutex> lk(g_dev_mutex);
     DeviceState& st = g_dev[dev];
     if (st.ws_bytes < bytes) {
@@ -685,7 +401,7 @@ static int ensure_workspace(int dev, size_t bytes, void** ptr) {
     return PCT_OK;
 }
 
-static size_t tiled_ws_bytes(const pct_problem* pb, int n_groups) {
+size_t tiled_ws_bytes(const pct_problem* pb, int n_groups) {
     const size_t N = (size_t)pb->nx * pb->ny * pb->nz;
     return ((2 * (size_t)n_groups * N * sizeof(float)) + 255) & ~(size_t)255;
 }
@@ -693,6 +409,54 @@ static size_t tiled_ws_bytes(const pct_problem* pb, int n_groups) {
 static bool is_tiled_choice(const pct_problem* pb, int choice) {
     return choice == PCT_CHOICE_SM && (pb->nh - 1) / 2 <= kMaxTiledHalfWindow &&
            (pb->nh - 1) / 2 >= 1;
+}
+
+int launch_finalize(const pct_problem* pb, const TileGeometry& g, const float* ws, float* out,
+                    hipStream_t stream) {
+    const long long N = (long long)pb->nx * pb->ny * pb->nz;
+    const dim3 fb(256), fgrid((pb->ny + 255) / 256, pb->nx, pb->nz);
+    hipLaunchKernelGGL(fuse_finalize, fgrid, fb, 0, stream, ws, g.n_items > 0 ? g.n_groups : 0, N,
+                       pb->nx, pb->ny, pb->nz, g.r, out);
+    return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;
+}
+
+TiledArgs make_tiled_args(const pct_problem* pb, const TileGeometry& g, const float* img,
+                          const float* atlas, const float* label, float* ws) {
+    TiledArgs a;
+    a.img = img;
+    a.atlas = atlas;
+    a.label = label;
+    a.ws = ws;
+    a.N = (long long)pb->nx * pb->ny * pb->nz;
+    a.nx = pb->nx;
+    a.ny = pb->ny;
+    a.nz = pb->nz;
+    a.r = g.r;
+    a.tiles_y = g.tiles_y;
+    a.tiles_x = g.tiles_x;
+    a.n_tiles = (int)g.n_tiles;
+    a.n_items = (int)g.n_items;
+    a.n_atlas = pb->n_atlas;
+    a.group = g.group;
+    const double div = pct_weight_divisor(pb->sigma, pb->beta);
+    a.scale = (float)std::sqrt(1.4426950408889634 / div);
+    a.only = nullptr;
+    return a;
+}
+
+// Launches the single-problem tiled kernel of `shape_id` over the items
+// described by `a` (no finalize pass).
+int launch_tiled_items(const TiledArgs& a, int shape_id, int hw, hipStream_t stream) {
+    const TiledEntry* e = find_entry(shape_id, hw);
+    if (!e) return PCT_ERR_UNSUPPORTED;
+    if (a.n_items <= 0) return PCT_OK;
+    TiledArgs copy = a;
+    void* kargs[] = {&copy};
+    if (hipLaunchKernel(reinterpret_cast<const void*>(e->fn), dim3((unsigned)a.n_items),
+                        dim3(e->shape.waves * 64), kargs, (size_t)e->lds_bytes,
+                        stream) != hipSuccess)
+        return PCT_ERR_HIP;
+    return PCT_OK;
 }
 
 // Optional per-launch HIP events around the fusion kernels (see
@@ -736,25 +500,7 @@ static int launch_tiled(const pct_problem* pb, const float* img, const float* at
     const TiledEntry* e = find_entry(shape_id, g.hw);
     if (!e) return PCT_ERR_UNSUPPORTED;
     if (ws_bytes < tiled_ws_bytes(pb, g.n_groups)) return PCT_ERR_INVALID;
-    const long long N = (long long)pb->nx * pb->ny * pb->nz;
-    TiledArgs a;
-    a.img = img;
-    a.atlas = atlas;
-    a.label = label;
-    a.ws = static_cast<float*>(ws);
-    a.N = N;
-    a.nx = pb->nx;
-    a.ny = pb->ny;
-    a.nz = pb->nz;
-    a.r = g.r;
-    a.tiles_y = g.tiles_y;
-    a.tiles_x = g.tiles_x;
-    a.n_tiles = (int)g.n_tiles;
-    a.n_items = (int)g.n_items;
-    a.n_atlas = pb->n_atlas;
-    a.group = g.group;
-    const double div = pct_weight_divisor(pb->sigma, pb->beta);
-    a.scale = (float)std::sqrt(1.4426950408889634 / div);
+    TiledArgs a = make_tiled_args(pb, g, img, atlas, label, static_cast<float*>(ws));
     if (g.n_items > 0) {
         void* kargs[] = {&a};
         TimedLaunch timed(stream);
@@ -763,14 +509,10 @@ static int launch_tiled(const pct_problem* pb, const float* img, const float* at
                             stream) != hipSuccess)
             return PCT_ERR_HIP;
     }
-    const dim3 fb(256), fgrid((pb->ny + 255) / 256, pb->nx, pb->nz);
-    hipLaunchKernelGGL(fuse_finalize, fgrid, fb, 0, stream, static_cast<const float*>(ws),
-                       g.n_items > 0 ? g.n_groups : 0, N, pb->nx, pb->ny, pb->nz, g.r, out);
-    if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
-    return PCT_OK;
+    return launch_finalize(pb, g, static_cast<const float*>(ws), out, stream);
 }
 
-static int launch_direct(const pct_problem* pb, const float* img, const float* atlas,
+int launch_direct(const pct_problem* pb, const float* img, const float* atlas,
                          const float* label, float* out, hipStream_t stream) {
     const long long N = (long long)pb->nx * pb->ny * pb->nz;
     const int hw = (pb->nh - 1) / 2;
@@ -1052,6 +794,7 @@ extern "C" int pct_autotune(const pct_problem* pb, const float* d_img, const flo
 }
 
 extern "C" void pct_release(void) {
+    release_batch_graphs();
     std::lock_guard<std::mutex> lk(g_dev_mutex);
     for (auto& kv : g_dev) {
         if (kv.second.ws) {

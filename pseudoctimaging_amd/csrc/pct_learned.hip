@@ -1,0 +1,810 @@
+// Learned candidate proposal for gfx950: embedding MLP on MFMA, per-tile
+// candidate selection, exact FP32 weighting of the kept candidates with the
+// tiled box-sum machinery, a tail term for the rest, and the full search as
+// fallback.  No counterpart in the reference program, which always weighs
+// all n*NH^3 candidates (src/CPU/NewPatchedBrainPureC.cpp:1399-1432).
+//
+//  embed_kernel   features -> 30x64 -> ReLU -> 64x64 -> ReLU -> 64x16, BF16
+//                 MFMA 16x16x16 with FP32 accumulation.  The chain runs
+//                 transposed (H^T = W^T X^T, voxels on the N axis): the
+//                 16x16 output tile of one layer is, lane for lane, the B
+//                 operand of the next layer, so no data moves between layers;
+//                 the 28 weight fragments stay in VGPRs.
+//  box_pass       NH-wide box sums of the atlas CT (three separable passes).
+//  select_kernel  per (tile, atlas): embedding distance of every window
+//                 offset summed over a 4x2x2 probe cluster at the tile centre
+//                 (atlas embeddings of the cluster's window staged in LDS),
+//                 bitonic sort, the topk offsets kept, a low score gap
+//                 flags the item for the full search.
+//  pruned_kernel  per (tile, atlas): the kept offsets and a 3x3x3 lattice of
+//                 sample offsets, weighted exactly with the same LDS slabs and
+//                 separable box sums as the plain tiled kernel; the other
+//                 candidates contribute (CT window sum - CT of the evaluated
+//                 candidates) times the mean exact weight of the samples.
+//  fallback       the plain tiled kernel over the flagged items only.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "pct_internal.h"
+#include "pct_mlp.h"
+#include "pct_tiled.h"
+
+namespace pct {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kFrags = 28;         // 8 (layer 1) + 16 (layer 2) + 4 (layer 3)
+constexpr int kSamples = 27;       // 3x3x3 lattice of tail samples
+constexpr int kMaxTopk = 64;
+constexpr int kMaxOffsets = kMaxTopk + kSamples;
+
+// ---------------------------------------------------------------------------
+// embedding
+// ---------------------------------------------------------------------------
+
+struct EmbedArgs {
+    const float* vol;               // [nvol][nz][nx][ny]
+    unsigned short* emb;            // [nvol][nz][nx][ny][16] BF16
+    const v4s* frag;                // [28][64] weight fragments
+    const float* bias;              // b1[64], b2[64], b3[16]
+    int nx, ny, nz, nvol;
+    float mu, inv_sd;
+    long long chunks;               // nvol * nz * nx * ceil(ny / 16)
+};
+
+__device__ __forceinline__ v4s relu_bf16(v4f h) {
+    v4s r;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r[j] = (short)f2bf(fmaxf(h[j], 0.f));
+    return r;
+}
+
+// One wave embeds 16 consecutive voxels along y per iteration.  Lane l:
+// voxel n = l % 16 of the chunk, feature/hidden group g = l / 16.
+__global__ __launch_bounds__(256) void embed_kernel(EmbedArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, n = lane & 15;
+    v4s w1[4][2], w2[4][4], w3[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) w1[t][s] = a.frag[(t * 2 + s) * 64 + lane];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) w2[t][s] = a.frag[(8 + t * 4 + s) * 64 + lane];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) w3[s] = a.frag[(24 + s) * 64 + lane];
+    v4f b1[4], b2[4], b3;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            b1[t][j] = a.bias[16 * t + 4 * g + j];
+            b2[t][j] = a.bias[64 + 16 * t + 4 * g + j];
+        }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b3[j] = a.bias[128 + 4 * g + j];
+
+    const int ychunks = (a.ny + 15) / 16;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    const long long vsize = (long long)a.nx * a.ny * a.nz;
+    for (long long c = wave; c < a.chunks; c += nwaves) {
+        const int yc = (int)(c % ychunks);
+        long long row = c / ychunks;
+        const int x = (int)(row % a.nx);
+        row /= a.nx;
+        const int z = (int)(row % a.nz);
+        const int vi = (int)(row / a.nz);
+        const int y = yc * 16 + n;
+        const float* __restrict__ v = a.vol + vi * vsize;
+        v4s xb[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                xb[s][j] = (short)f2bf(
+                    feature(v, a.nx, a.ny, a.nz, z, x, y, 16 * s + 4 * g + j, a.mu, a.inv_sd));
+        v4f h1[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            v4f acc = b1[t];
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[t][0], xb[0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[t][1], xb[1], acc, 0, 0, 0);
+            h1[t] = acc;
+        }
+        v4s h1b[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) h1b[t] = relu_bf16(h1[t]);
+        v4f h2[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            v4f acc = b2[t];
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2[t][s], h1b[s], acc, 0, 0, 0);
+            h2[t] = acc;
+        }
+        v4s h2b[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) h2b[t] = relu_bf16(h2[t]);
+        v4f e = b3;
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            e = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3[s], h2b[s], e, 0, 0, 0);
+        if (y < a.ny) {
+            const unsigned lo = (unsigned)f2bf(e[0]) | ((unsigned)f2bf(e[1]) << 16);
+            const unsigned hi = (unsigned)f2bf(e[2]) | ((unsigned)f2bf(e[3]) << 16);
+            uint2* dst = reinterpret_cast<uint2*>(
+                a.emb + ((vi * vsize + ((long long)z * a.nx + x) * a.ny + y) * 16 + 4 * g));
+            *dst = make_uint2(lo, hi);
+        }
+    }
+}
+
+// Weight fragments in MFMA A-operand order: fragment (layer, M-tile t,
+// K-step s), lane l, element j holds W[k = 16 s + 4 (l / 16) + j][m = 16 t + l % 16].
+static void make_fragments(const float* p, std::vector<unsigned short>& frag,
+                           std::vector<float>& bias) {
+    auto bf = [](float f) {
+        unsigned u;
+        std::memcpy(&u, &f, 4);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return (unsigned short)(u >> 16);
+    };
+    frag.assign(kFrags * 64 * 4, 0);
+    auto put = [&](int fi, const float* W, int ncols, int t, int s) {
+        for (int l = 0; l < 64; ++l)
+            for (int j = 0; j < 4; ++j) {
+                const int k = 16 * s + 4 * (l / 16) + j, m = 16 * t + l % 16;
+                frag[(fi * 64 + l) * 4 + j] = bf(W[k * ncols + m]);
+            }
+    };
+    for (int t = 0; t < 4; ++t)
+        for (int s = 0; s < 2; ++s) put(t * 2 + s, p + kOffW1, 64, t, s);
+    for (int t = 0; t < 4; ++t)
+        for (int s = 0; s < 4; ++s) put(8 + t * 4 + s, p + kOffW2, 64, t, s);
+    for (int s = 0; s < 4; ++s) put(24 + s, p + kOffW3, 16, 0, s);
+    bias.assign(144, 0.f);
+    for (int i = 0; i < 64; ++i) bias[i] = p[kOffB1 + i];
+    for (int i = 0; i < 64; ++i) bias[64 + i] = p[kOffB2 + i];
+    for (int i = 0; i < 16; ++i) bias[128 + i] = p[kOffB3 + i];
+}
+
+// Uploads the fragments and biases to `dst` (kFrags*64*8 + 144*4 bytes).
+static int upload_model(const float* params, void* dst, hipStream_t stream) {
+    std::vector<unsigned short> frag;
+    std::vector<float> bias;
+    make_fragments(params, frag, bias);
+    char* d = static_cast<char*>(dst);
+    if (hipMemcpyAsync(d, frag.data(), frag.size() * 2, hipMemcpyHostToDevice, stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(d + frag.size() * 2, bias.data(), bias.size() * 4, hipMemcpyHostToDevice,
+                       stream) != hipSuccess)
+        return PCT_ERR_HIP;
+    // the host vectors die here: make the copies complete first
+    return hipStreamSynchronize(stream) == hipSuccess ? PCT_OK : PCT_ERR_HIP;
+}
+constexpr size_t kModelBytes = kFrags * 64 * 8 + 144 * 4;
+
+static int launch_embed(const float* params, const void* d_model, const float* vol, int nx,
+                        int ny, int nz, int nvol, unsigned short* emb, hipStream_t stream) {
+    EmbedArgs a;
+    a.vol = vol;
+    a.emb = emb;
+    a.frag = static_cast<const v4s*>(d_model);
+    a.bias = reinterpret_cast<const float*>(static_cast<const char*>(d_model) + kFrags * 64 * 8);
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.nvol = nvol;
+    a.mu = params[kOffMu];
+    a.inv_sd = params[kOffSd];
+    a.chunks = (long long)nvol * nz * nx * ((ny + 15) / 16);
+    const long long waves = std::min<long long>(a.chunks, 256LL * 32);
+    const unsigned blocks = (unsigned)((waves + 3) / 4);
+    hipLaunchKernelGGL(embed_kernel, dim3(blocks), dim3(256), 0, stream, a);
+    return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;
+}
+
+// ---------------------------------------------------------------------------
+// window box sums of the atlas CT
+// ---------------------------------------------------------------------------
+
+// out = sum of in over [c - hw, c + hw] along `axis` (0 z, 1 x, 2 y), zero
+// outside the volume; nvol volumes one after the other.
+__global__ void box_pass(const float* __restrict__ in, float* __restrict__ out, int nx, int ny,
+                         int nz, int nvol, int hw, int axis) {
+    const long long vsize = (long long)nx * ny * nz;
+    const long long total = vsize * nvol;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long vi = i / vsize;
+        const long long r = i - vi * vsize;
+        const int y = (int)(r % ny);
+        const int x = (int)((r / ny) % nx);
+        const int z = (int)(r / ((long long)nx * ny));
+        const float* __restrict__ v = in + vi * vsize;
+        float s = 0.f;
+        if (axis == 2) {
+            const int lo = max(0, y - hw), hi = min(ny - 1, y + hw);
+            for (int k = lo; k <= hi; ++k) s += v[((long long)z * nx + x) * ny + k];
+        } else if (axis == 1) {
+            const int lo = max(0, x - hw), hi = min(nx - 1, x + hw);
+            for (int k = lo; k <= hi; ++k) s += v[((long long)z * nx + k) * ny + y];
+        } else {
+            const int lo = max(0, z - hw), hi = min(nz - 1, z + hw);
+            for (int k = lo; k <= hi; ++k) s += v[((long long)k * nx + x) * ny + y];
+        }
+        out[i] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// candidate selection
+// ---------------------------------------------------------------------------
+
+struct SelectArgs {
+    const unsigned short* embT;   // [N][16]
+    const unsigned short* embA;   // [n][N][16]
+    int* sel;                     // [items][topk] kept offset indices, ascending
+    unsigned char* flag;          // [items] 1 = full search
+    long long N;
+    int nx, ny, nz, hw, r;
+    int tiles_y, tiles_x, n_tiles;
+    int oy, wx, bz;               // tile extents in y, x, z
+    int topk;
+    float margin;
+};
+
+constexpr int kSelThreads = 256;
+constexpr int kSelPerThread = 16;   // 4096 sort slots (NH = 15: 3375 offsets)
+
+// One workgroup per item = (atlas, tile), item = atlas * n_tiles + tile.
+__global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
+    extern __shared__ float sh[];
+    const int item = blockIdx.x;
+    const int at = item / a.n_tiles;
+    int t = item - at * a.n_tiles;
+    const int ty = t % a.tiles_y;
+    t /= a.tiles_y;
+    const int tx = t % a.tiles_x;
+    const int tz = t / a.tiles_x;
+    const int r = a.r, hw = a.hw, nw = 2 * hw + 1;
+    const int ncand = nw * nw * nw;
+    // probe cluster: 4 (y) x 2 (x) x 2 (z) voxels at the centre of the
+    // tile's part of the interior
+    const int y0 = r + ty * a.oy, x0 = r + tx * a.wx, z0 = r + tz * a.bz;
+    const int ye = min(y0 + a.oy, a.ny - r), xe = min(x0 + a.wx, a.nx - r),
+              ze = min(z0 + a.bz, a.nz - r);
+    const int py0 = max(r, min((y0 + ye) / 2 - 2, a.ny - r - 4));
+    const int px0 = max(r, min((x0 + xe) / 2 - 1, a.nx - r - 2));
+    const int pz0 = max(r, min((z0 + ze) / 2 - 1, a.nz - r - 2));
+    // LDS: probe target embeddings (16 x 16 floats) and the atlas embeddings
+    // of the cluster's window (BF16 pairs); afterwards the sort keys/values.
+    // Coordinates are clamped into the volume (a thin interior can leave the
+    // cluster's window partly outside).
+    float* sT = sh;
+    unsigned* sA = reinterpret_cast<unsigned*>(sh + 256);
+    const int WY = 4 + 2 * hw, WXX = 2 + 2 * hw, WZ = 2 + 2 * hw;
+    const int nwin = WY * WXX * WZ;
+    const long long plane = (long long)a.nx * a.ny;
+    for (int i = threadIdx.x; i < 256; i += kSelThreads) {
+        const int p = i >> 4, k = i & 15;
+        const int yy = min(py0 + (p & 3), a.ny - 1), xx = min(px0 + ((p >> 2) & 1), a.nx - 1),
+                  zz = min(pz0 + (p >> 3), a.nz - 1);
+        sT[i] = bf2f(a.embT[(zz * plane + (long long)xx * a.ny + yy) * 16 + k]);
+    }
+    const unsigned* __restrict__ eA =
+        reinterpret_cast<const unsigned*>(a.embA + (long long)at * a.N * 16);
+    for (int i = threadIdx.x; i < nwin * 8; i += kSelThreads) {
+        const int w = i >> 3, k2 = i & 7;
+        const int wy = w % WY, wx = (w / WY) % WXX, wz = w / (WY * WXX);
+        const int yy = min(max(py0 - hw + wy, 0), a.ny - 1);
+        const int xx = min(max(px0 - hw + wx, 0), a.nx - 1);
+        const int zz = min(max(pz0 - hw + wz, 0), a.nz - 1);
+        sA[i] = eA[(zz * plane + (long long)xx * a.ny + yy) * 8 + k2];
+    }
+    __syncthreads();
+    // scores: thread handles offsets c = threadIdx.x + q * kSelThreads
+    int p2 = 1;
+    while (p2 < ncand) p2 <<= 1;
+    float sc[kSelPerThread];
+#pragma unroll
+    for (int q = 0; q < kSelPerThread; ++q) {
+        const int c = threadIdx.x + q * kSelThreads;
+        float score = __builtin_huge_valf();
+        if (c < ncand) {
+            const int dz = c / (nw * nw) - hw, dx = (c / nw) % nw - hw, dy = c % nw - hw;
+            score = 0.f;
+            for (int p = 0; p < 16; ++p) {
+                const int wy = (p & 3) + hw + dy, wx = ((p >> 2) & 1) + hw + dx,
+                          wz = (p >> 3) + hw + dz;
+                const unsigned* ea = sA + ((wz * WXX + wx) * WY + wy) * 8;
+                const float* et = sT + p * 16;
+#pragma unroll
+                for (int k2 = 0; k2 < 8; ++k2) {
+                    const unsigned u = ea[k2];
+                    const float d0 = et[2 * k2] - __uint_as_float(u << 16);
+                    const float d1 = et[2 * k2 + 1] - __uint_as_float(u & 0xffff0000u);
+                    score = fmaf(d0, d0, score);
+                    score = fmaf(d1, d1, score);
+                }
+            }
+        }
+        sc[q] = score;
+    }
+    __syncthreads();   // the window is dead from here on
+    float* key = sh;
+    int* val = reinterpret_cast<int*>(sh + p2);
+#pragma unroll
+    for (int q = 0; q < kSelPerThread; ++q) {
+        const int c = threadIdx.x + q * kSelThreads;
+        if (c < p2) {
+            key[c] = sc[q];
+            val[c] = c;
+        }
+    }
+    __syncthreads();
+    // bitonic sort ascending by (score, offset index)
+    for (int size = 2; size <= p2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < p2 / 2; i += kSelThreads) {
+                const int lo = 2 * i - (i & (stride - 1));
+                const int hi = lo + stride;
+                const bool up = ((lo & size) == 0);
+                const float kl = key[lo], kh = key[hi];
+                const int vl = val[lo], vh = val[hi];
+                const bool gt = kl > kh || (kl == kh && vl > vh);
+                if (gt == up) {
+                    key[lo] = kh;
+                    key[hi] = kl;
+                    val[lo] = vh;
+                    val[hi] = vl;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const int k = min(a.topk, ncand);
+    if (threadIdx.x == 0) {
+        // a small gap between the last kept and the first dropped score,
+        // relative to the spread of the scores, flags the item
+        int f = a.margin < 0.f ? 1 : 0;
+        if (!f && k < ncand) {
+            const float spread = key[ncand / 2] - key[0];
+            const float gap = key[k] - key[k - 1];
+            f = (spread > 0.f && gap < a.margin * spread) ? 1 : 0;
+        }
+        a.flag[item] = (unsigned char)f;
+    }
+    // kept offsets in ascending index order (dz-major), by rank counting
+    for (int i = threadIdx.x; i < k; i += kSelThreads) {
+        const int v = val[i];
+        int rank = 0;
+        for (int j = 0; j < k; ++j) rank += val[j] < v;
+        a.sel[(long long)item * a.topk + rank] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pruned exact weighting
+// ---------------------------------------------------------------------------
+
+struct PrunedArgs {
+    TiledArgs t;                    // problem, tiles (group = 1), workspace partials
+    const int* sel;                 // [items][topk]
+    const unsigned char* flag;      // [items]
+    const float* boxL;              // [n][N] window sums of the CT
+    int topk;
+    int n_samples;
+    int samples[kSamples];          // lattice offset indices, ascending
+    int ncand;                      // NH^3
+};
+
+// As tiled_step, also summing the CT of the evaluated candidates (Le) and
+// the weights of the lattice samples (Ws).
+template <class C, int BX, int BZ, bool SAMPLE>
+__device__ __forceinline__ void pruned_step(const float* __restrict__ lds, int base,
+                                            const float (&tv)[BZ + 2][BX + 2],
+                                            float (&P)[BZ][BX], float (&S)[BZ][BX],
+                                            float (&Le)[BZ][BX], float (&Ws)[BZ][BX]) {
+    constexpr int NBA = (BZ + 2 + C::PPB - 1) / C::PPB;
+    constexpr int NBL = (BZ + C::PPB - 1) / C::PPB;
+    int ba[NBA];
+    int bl[NBL];
+#pragma unroll
+    for (int b = 0; b < NBA; ++b) ba[b] = b ? opaque(base + b * C::PPB * C::PS) : base;
+#pragma unroll
+    for (int b = 0; b < NBL; ++b) bl[b] = opaque(base + C::LOFF + b * C::PPB * C::PS);
+    float ex[BZ + 2][BX];
+#pragma unroll
+    for (int pz = 0; pz < BZ + 2; ++pz) {
+        float d[BX + 2];
+        const int pp = ba[pz / C::PPB] + (pz % C::PPB) * C::PS;
+#pragma unroll
+        for (int i = 0; i < BX + 2; ++i) {
+            const float df = tv[pz][i] - lds[pp + i * C::CW];
+            d[i] = df * df;
+        }
+#pragma unroll
+        for (int i = 0; i < BX; i += 2) {
+            const float q = d[i + 1] + d[i + 2];
+            ex[pz][i] = d[i] + q;
+            ex[pz][i + 1] = q + d[i + 3];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < BZ; j += 2) {
+#pragma unroll
+        for (int i = 0; i < BX; ++i) {
+            const float q = ex[j + 1][i] + ex[j + 2][i];
+            const float e0 = ex[j][i] + q;
+            const float e1 = q + ex[j + 3][i];
+            const float s0 = lane_next(e0) + (lane_prev(e0) + e0);
+            const float s1 = lane_next(e1) + (lane_prev(e1) + e1);
+            const float w0 = __builtin_amdgcn_exp2f(-s0);
+            const float w1 = __builtin_amdgcn_exp2f(-s1);
+            const float l0 = lds[bl[j / C::PPB] + (j % C::PPB) * C::PS + (i + 1) * C::CW];
+            const float l1 =
+                lds[bl[(j + 1) / C::PPB] + ((j + 1) % C::PPB) * C::PS + (i + 1) * C::CW];
+            P[j][i] = fmaf(w0, l0, P[j][i]);
+            S[j][i] += w0;
+            P[j + 1][i] = fmaf(w1, l1, P[j + 1][i]);
+            S[j + 1][i] += w1;
+            Le[j][i] += l0;
+            Le[j + 1][i] += l1;
+            if (SAMPLE) {
+                Ws[j][i] += w0;
+                Ws[j + 1][i] += w1;
+            }
+        }
+    }
+}
+
+template <int W, int BX, int BZ, int NW, int HW>
+__global__ __launch_bounds__(NW * 64) void pruned_kernel(PrunedArgs pa) {
+    using C = TiledCfg<W, BX, BZ, NW, HW>;
+    extern __shared__ float lds[];
+    __shared__ int offs[kMaxOffsets];     // merged offset list, ascending
+    __shared__ int is_sample[kMaxOffsets];
+    __shared__ int n_offs, n_samp_eff;
+    const TiledArgs& args = pa.t;
+    const int item = xcd_contiguous(blockIdx.x, args.n_items);
+    if (pa.flag[item]) return;            // full search handles it (uniform)
+    const int at = item / args.n_tiles;
+    int t = item - at * args.n_tiles;
+    const int ty = t % args.tiles_y;
+    t /= args.tiles_y;
+    const int tx = t % args.tiles_x;
+    const int tz = t / args.tiles_x;
+
+    if (threadIdx.x == 0) {
+        // merge the kept offsets with the lattice samples (both ascending);
+        // a sample that is also kept counts once, as kept
+        const int* ks = pa.sel + (long long)item * pa.topk;
+        int i = 0, j = 0, m = 0, ns = 0;
+        while (i < pa.topk || j < pa.n_samples) {
+            const int a = i < pa.topk ? ks[i] : 1 << 30;
+            const int b = j < pa.n_samples ? pa.samples[j] : 1 << 30;
+            if (a < b) {
+                offs[m] = a; is_sample[m] = 0; ++m; ++i;
+            } else if (b < a) {
+                offs[m] = b; is_sample[m] = 1; ++m; ++j; ++ns;
+            } else {
+                offs[m] = a; is_sample[m] = 0; ++m; ++i; ++j;
+            }
+        }
+        n_offs = m;
+        n_samp_eff = ns;
+    }
+    __syncthreads();
+
+    const int tid = threadIdx.x;
+    const int wv = tid >> 6;
+    const int lane = tid & 63;
+    const int seg = lane / W;
+    const int ly = lane % W;
+    const int sg = wv * C::S + seg;
+    const int nx = args.nx, ny = args.ny, nz = args.nz, r = args.r;
+    const long long plane = (long long)nx * ny;
+    const int y0 = r + ty * C::OY;
+    const int x0 = r + tx * C::WX;
+    const int z0 = r + tz * BZ;
+    const int y = y0 - 1 + ly;
+    const int xb = x0 + sg * BX;
+
+    float tv[BZ + 2][BX + 2];
+#pragma unroll
+    for (int pz = 0; pz < BZ + 2; ++pz)
+#pragma unroll
+        for (int i = 0; i < BX + 2; ++i) {
+            const int zz = z0 - 1 + pz, xx = xb - 1 + i;
+            const bool ok = zz < nz && xx < nx && y < ny;
+            tv[pz][i] = ok ? args.img[zz * plane + (long long)xx * ny + y] * args.scale : 0.f;
+        }
+    float P[BZ][BX], S[BZ][BX], Le[BZ][BX], Ws[BZ][BX];
+#pragma unroll
+    for (int j = 0; j < BZ; ++j)
+#pragma unroll
+        for (int i = 0; i < BX; ++i) P[j][i] = S[j][i] = Le[j][i] = Ws[j][i] = 0.f;
+
+    const int lane_base = sg * BX * C::CW + ly;
+    const int xs = x0 - 1 - HW;
+    const int ys = y0 - 1 - HW;
+    constexpr int NWIN = 2 * HW + 1;
+    const float* __restrict__ A = args.atlas + (long long)at * args.N;
+    const float* __restrict__ L = args.label + (long long)at * args.N;
+    const int m = n_offs;
+    int k = 0;
+    while (k < m) {
+        const int dz = offs[k] / (NWIN * NWIN) - HW;
+        __syncthreads();
+        stage_slab<C, NW>(lds, A, L, args.scale, z0 + dz, xs, ys, nx, ny, nz, plane, wv, lane);
+        __syncthreads();
+        for (; k < m && offs[k] / (NWIN * NWIN) - HW == dz; ++k) {
+            const int dx = (offs[k] / NWIN) % NWIN, dy = offs[k] % NWIN;   // 0 .. 2HW
+            const int base = lane_base + dx * C::CW + dy;
+            if (is_sample[k])
+                pruned_step<C, BX, BZ, true>(lds, base, tv, P, S, Le, Ws);
+            else
+                pruned_step<C, BX, BZ, false>(lds, base, tv, P, S, Le, Ws);
+        }
+    }
+
+    // tail: the candidates not evaluated, at the mean sample weight
+    if (ly >= 1 && ly <= W - 2 && y < ny - r) {
+        const int ns = n_samp_eff;
+        const float n_rest = (float)(pa.ncand - m);
+        float* __restrict__ wsP = args.ws + (long long)(2 * at) * args.N;
+        float* __restrict__ wsS = wsP + args.N;
+        const float* __restrict__ bL = pa.boxL + (long long)at * args.N;
+#pragma unroll
+        for (int j = 0; j < BZ; ++j)
+#pragma unroll
+            for (int i = 0; i < BX; ++i) {
+                const int z = z0 + j, x = xb + i;
+                if (z < nz - r && x < nx - r) {
+                    const long long idx = z * plane + (long long)x * ny + y;
+                    const float wbar = ns > 0 ? Ws[j][i] / (float)ns : 0.f;
+                    wsP[idx] = fmaf(wbar, bL[idx] - Le[j][i], P[j][i]);
+                    wsS[idx] = fmaf(wbar, n_rest, S[j][i]);
+                }
+            }
+    }
+}
+
+using PrunedFn = void (*)(PrunedArgs);
+
+struct PrunedEntry {
+    int shape_id;
+    PrunedFn fn[kMaxTiledHalfWindow];
+};
+
+template <int W, int BX, int BZ, int NW>
+PrunedEntry make_pruned(int shape_id) {
+    return PrunedEntry{shape_id,
+                       {&pruned_kernel<W, BX, BZ, NW, 1>, &pruned_kernel<W, BX, BZ, NW, 2>,
+                        &pruned_kernel<W, BX, BZ, NW, 3>, &pruned_kernel<W, BX, BZ, NW, 4>,
+                        &pruned_kernel<W, BX, BZ, NW, 5>, &pruned_kernel<W, BX, BZ, NW, 6>,
+                        &pruned_kernel<W, BX, BZ, NW, 7>}};
+}
+
+static const std::vector<PrunedEntry>& pruned_entries() {
+    static const std::vector<PrunedEntry> e = {
+        make_pruned<32, 4, 2, 4>(8),
+        make_pruned<32, 2, 4, 8>(13),
+    };
+    return e;
+}
+
+static const PrunedEntry* find_pruned(int shape_id) {
+    for (const auto& e : pruned_entries())
+        if (e.shape_id == shape_id) return &e;
+    return nullptr;
+}
+
+}  // namespace pct
+
+using namespace pct;
+
+extern "C" int pct_mlp_init(float* params, uint64_t seed, float mu, float inv_sd) {
+    if (!params || !(inv_sd > 0.f)) return PCT_ERR_INVALID;
+    std::memset(params, 0, sizeof(float) * PCT_MLP_NPARAMS);
+    uint64_t st = seed ^ 0x5DEECE66DULL;
+    auto normal = [&]() {
+        const double u1 = ((double)(splitmix64(st) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        const double u2 = (double)(splitmix64(st) >> 11) * (1.0 / 9007199254740992.0);
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    for (int k = 0; k < 30; ++k)
+        for (int m = 0; m < 64; ++m) params[kOffW1 + k * 64 + m] = (float)(normal() * std::sqrt(2.0 / 30));
+    for (int k = 0; k < 64; ++k)
+        for (int m = 0; m < 64; ++m) params[kOffW2 + k * 64 + m] = (float)(normal() * std::sqrt(2.0 / 64));
+    for (int k = 0; k < 64; ++k)
+        for (int m = 0; m < 16; ++m) params[kOffW3 + k * 16 + m] = (float)(normal() * std::sqrt(1.0 / 64));
+    params[kOffMu] = mu;
+    params[kOffSd] = inv_sd;
+    return PCT_OK;
+}
+
+extern "C" int pct_embed_dev(const float* params, const float* d_vol, int nx, int ny, int nz,
+                             uint16_t* d_emb, pct_stream_t stream) {
+    if (!params || !d_vol || !d_emb || nx < 1 || ny < 1 || nz < 1) return PCT_ERR_INVALID;
+    int dev = 0;
+    int rc = check_device(&dev);
+    if (rc) return rc;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    void* ws = nullptr;
+    rc = ensure_workspace(dev, kModelBytes + 256, &ws);
+    if (rc) return rc;
+    rc = upload_model(params, ws, s);
+    if (rc) return rc;
+    return launch_embed(params, ws, d_vol, nx, ny, nz, 1, reinterpret_cast<unsigned short*>(d_emb), s);
+}
+
+extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
+                                   const pct_pruned_opts* opts, const float* d_img,
+                                   const float* d_atlas, const float* d_label, float* d_out,
+                                   double* fallback_frac, pct_stream_t stream) {
+    int rc = validate_problem(pb);
+    if (rc) return rc;
+    if (!params || !opts || !d_img || !d_atlas || !d_label || !d_out) return PCT_ERR_INVALID;
+    if (opts->topk < 1 || opts->topk > kMaxTopk || opts->samples < 0) return PCT_ERR_INVALID;
+    const int hw = (pb->nh - 1) / 2;
+    if (hw < 1 || hw > kMaxTiledHalfWindow) return PCT_ERR_UNSUPPORTED;
+    int dev = 0;
+    rc = check_device(&dev);
+    if (rc) return rc;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int shape_id = opts->shape_id >= 0 ? opts->shape_id : 13;
+    const PrunedEntry* pe = find_pruned(shape_id);
+    if (!pe) return PCT_ERR_UNSUPPORTED;
+    TileGeometry g;
+    if (!make_geometry(pb, shape_id, 1, &g)) return PCT_ERR_UNSUPPORTED;
+    const TileShape& sh = shapes()[shape_id];
+    const int n = pb->n_atlas;
+    const long long N = (long long)pb->nx * pb->ny * pb->nz;
+    const long long items = g.n_tiles * n;
+    const int ncand = (2 * hw + 1) * (2 * hw + 1) * (2 * hw + 1);
+    const int topk = std::min(opts->topk, ncand);
+
+    // workspace layout
+    auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    size_t off = 0;
+    const size_t o_model = off; off = align(off + kModelBytes);
+    const size_t o_part = off; off = align(off + sizeof(float) * 2 * (size_t)n * N);
+    const size_t o_embT = off; off = align(off + 32 * (size_t)N);
+    const size_t o_embA = off; off = align(off + 32 * (size_t)N * n);
+    const size_t o_boxL = off; off = align(off + sizeof(float) * (size_t)N * n);
+    const size_t o_tmp = off; off = align(off + sizeof(float) * (size_t)N * n);
+    const size_t o_sel = off; off = align(off + sizeof(int) * (size_t)items * topk);
+    const size_t o_flag = off; off = align(off + (size_t)items);
+    void* ws = nullptr;
+    rc = ensure_workspace(dev, off, &ws);
+    if (rc) return rc;
+    char* w = static_cast<char*>(ws);
+    rc = upload_model(params, w + o_model, s);
+    if (rc) return rc;
+
+    // 1. embeddings of the target and of every atlas MRI
+    unsigned short* embT = reinterpret_cast<unsigned short*>(w + o_embT);
+    unsigned short* embA = reinterpret_cast<unsigned short*>(w + o_embA);
+    rc = launch_embed(params, w + o_model, d_img, pb->nx, pb->ny, pb->nz, 1, embT, s);
+    if (!rc) rc = launch_embed(params, w + o_model, d_atlas, pb->nx, pb->ny, pb->nz, n, embA, s);
+    if (rc) return rc;
+
+    // 2. window sums of every atlas CT: y into tmp, x into boxL, z into tmp
+    float* boxL = reinterpret_cast<float*>(w + o_boxL);
+    float* tmp = reinterpret_cast<float*>(w + o_tmp);
+    const unsigned bb = (unsigned)std::min<long long>((N * n + 255) / 256, 65536);
+    hipLaunchKernelGGL(box_pass, dim3(bb), dim3(256), 0, s, d_label, tmp, pb->nx, pb->ny, pb->nz,
+                       n, hw, 2);
+    hipLaunchKernelGGL(box_pass, dim3(bb), dim3(256), 0, s, tmp, boxL, pb->nx, pb->ny, pb->nz, n,
+                       hw, 1);
+    hipLaunchKernelGGL(box_pass, dim3(bb), dim3(256), 0, s, boxL, tmp, pb->nx, pb->ny, pb->nz, n,
+                       hw, 0);
+    if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
+    std::swap(boxL, tmp);
+
+    // 3. selection
+    SelectArgs sa;
+    sa.embT = embT;
+    sa.embA = embA;
+    sa.sel = reinterpret_cast<int*>(w + o_sel);
+    sa.flag = reinterpret_cast<unsigned char*>(w + o_flag);
+    sa.N = N;
+    sa.nx = pb->nx;
+    sa.ny = pb->ny;
+    sa.nz = pb->nz;
+    sa.hw = hw;
+    sa.r = g.r;
+    sa.tiles_y = g.tiles_y;
+    sa.tiles_x = g.tiles_x;
+    sa.n_tiles = (int)g.n_tiles;
+    sa.oy = sh.seg_width - 2;
+    sa.wx = sh.waves * (64 / sh.seg_width) * sh.bx;
+    sa.bz = sh.bz;
+    sa.topk = topk;
+    sa.margin = opts->margin;
+    int p2 = 1;
+    while (p2 < ncand) p2 <<= 1;
+    if (p2 > kSelThreads * kSelPerThread) return PCT_ERR_UNSUPPORTED;
+    const int win = (4 + 2 * hw) * (2 + 2 * hw) * (2 + 2 * hw);
+    const size_t sel_lds = std::max(256 * 4 + (size_t)win * 32, (size_t)p2 * 8);
+    if (sel_lds > (size_t)kLdsPerCu) return PCT_ERR_UNSUPPORTED;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(select_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sel_lds);
+    if (items > 0) {
+        hipLaunchKernelGGL(select_kernel, dim3((unsigned)items), dim3(kSelThreads), sel_lds, s, sa);
+        if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
+    }
+
+    // 4. exact weighting of the kept candidates and the lattice samples
+    float* part = reinterpret_cast<float*>(w + o_part);
+    PrunedArgs pa;
+    std::memset(&pa, 0, sizeof(pa));
+    TileGeometry g1 = g;
+    g1.group = 1;
+    g1.n_groups = n;
+    g1.n_items = items;
+    pa.t = make_tiled_args(pb, g1, d_img, d_atlas, d_label, part);
+    pa.sel = sa.sel;
+    pa.flag = sa.flag;
+    pa.boxL = boxL;
+    pa.topk = topk;
+    pa.ncand = ncand;
+    {
+        // 3x3x3 lattice at 0 and +-ceil(2 hw / 3)
+        const int q = std::max(1, (2 * hw + 2) / 3);
+        const int lat[3] = {-q, 0, q};
+        int ns = 0;
+        for (int a = 0; a < 3 && opts->samples > 0; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int c = 0; c < 3; ++c) {
+                    const int nw = 2 * hw + 1;
+                    if (ns < std::min(opts->samples, kSamples))
+                        pa.samples[ns++] =
+                            ((lat[a] + hw) * nw + (lat[b] + hw)) * nw + (lat[c] + hw);
+                }
+        std::sort(pa.samples, pa.samples + ns);
+        pa.n_samples = ns;
+    }
+    PrunedFn fn = pe->fn[hw - 1];
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, g.lds_bytes);
+    if (items > 0) {
+        void* kargs[] = {&pa};
+        if (hipLaunchKernel(reinterpret_cast<const void*>(fn), dim3((unsigned)items),
+                            dim3(sh.waves * 64), kargs, (size_t)g.lds_bytes, s) != hipSuccess)
+            return PCT_ERR_HIP;
+    }
+
+    // 5. full exact search for the flagged items
+    TiledArgs fa = pa.t;
+    fa.only = sa.flag;
+    rc = launch_tiled_items(fa, shape_id, hw, s);
+    if (rc) return rc;
+
+    // 6. sum over atlases, NaN rule, zero border
+    rc = launch_finalize(pb, g1, part, d_out, s);
+    if (rc) return rc;
+
+    if (fallback_frac) {
+        std::vector<unsigned char> fl((size_t)items);
+        if (items > 0 && hipMemcpyAsync(fl.data(), sa.flag, (size_t)items, hipMemcpyDeviceToHost,
+                                        s) != hipSuccess)
+            return PCT_ERR_HIP;
+        if (hipStreamSynchronize(s) != hipSuccess) return PCT_ERR_HIP;
+        long long cnt = 0;
+        for (unsigned char c : fl) cnt += c;
+        *fallback_frac = items > 0 ? (double)cnt / (double)items : 0.0;
+    }
+    return PCT_OK;
+}
