@@ -110,7 +110,7 @@ def batched_leg(steps: int, warmup: int, seed: int):
 
 def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup, seed,
                 plain_ms, train_shape=(64, 96, 96), train_subjects=4, train_rounds=2,
-                train_steps=60, topk=32, samples=27, margin=0.0):
+                train_steps=60, topk=32, samples=27, margin=0.0, max_slabs=4):
     """Learned-pruned fusion on the resident inputs: the embedding MLP is
     trained on the GPU on `train_subjects` other seeded phantoms, then the
     pruned fusion is timed and compared with the plain output."""
@@ -139,16 +139,16 @@ def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup,
     train_s = time.perf_counter() - t0
     del subjects
     out = torch.empty_like(plain_out)
-    frac = learned.fuse_pruned_device(d_img, d_amri, d_lab, out, nh, sigma, beta, params,
-                                      topk=topk, samples=samples, margin=margin)
+    kw = dict(topk=topk, samples=samples, margin=margin, max_slabs=max_slabs)
+    frac = learned.fuse_pruned_device(d_img, d_amri, d_lab, out, nh, sigma, beta, params, **kw)
     for _ in range(warmup):
-        learned.fuse_pruned_device(d_img, d_amri, d_lab, out, nh, sigma, beta, params, topk=topk,
-                                   samples=samples, margin=margin, report_fallback=False)
+        learned.fuse_pruned_device(d_img, d_amri, d_lab, out, nh, sigma, beta, params,
+                                   report_fallback=False, **kw)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        learned.fuse_pruned_device(d_img, d_amri, d_lab, out, nh, sigma, beta, params, topk=topk,
-                                   samples=samples, margin=margin, report_fallback=False)
+        learned.fuse_pruned_device(d_img, d_amri, d_lab, out, nh, sigma, beta, params,
+                                   report_fallback=False, **kw)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
     a = out.cpu().numpy().astype(np.float64)
@@ -162,6 +162,7 @@ def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup,
             float(np.percentile(dhu, 99)),
             "nan_mismatch": int(np.count_nonzero(np.isnan(a) != np.isnan(b))),
             "fallback_frac": frac, "topk": topk, "lattice_samples": samples, "margin": margin,
+            "max_slabs": max_slabs,
             "embedding": "30-64-64-16 MLP, BF16 MFMA inference, FP32 training on GPU",
             "train": {"subjects": train_subjects, "shape_zxy": list(train_shape),
                       "rounds": train_rounds, "steps_per_subject": train_steps,
@@ -189,6 +190,9 @@ def main() -> int:
     ap.add_argument("--parity-voxels", type=int, default=2000)
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-pruned leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized float32 buffer three times (a plain copy "
+                         "kernel of known size, to scale FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -200,6 +204,17 @@ def main() -> int:
 
     import numpy as np
     import torch
+
+    if args.copy_calibration > 0:
+        nbytes = args.copy_calibration << 20
+        a = torch.ones(nbytes // 4, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": nbytes, "bytes_written": nbytes,
+                          "dispatches": 3}), flush=True)
+        return 0
 
     import pseudoctimaging_amd as pct
     from pseudoctimaging_amd import _lib
@@ -274,7 +289,15 @@ def main() -> int:
     tfile = os.path.join(ROOT, "profiles", "fuse_tiled_222x222x112_n10_nh11_traffic.json")
     if (nz, nx, ny, n, nh) == (112, 222, 222, 10, 11) and os.path.exists(tfile):
         with open(tfile) as f:
-            roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f)
+        roofline["traffic"] = prof.get("hbm_bytes_per_launch")
+        if "valu" in prof:
+            v = prof["valu"]
+            roofline["counters"] = {
+                "valu_issue_fraction": v["valu_issue_fraction"],
+                "executed_fp32_tflops": round(v["executed_fp32_flop_per_launch"]
+                                              / (kernel_ms * 1e-3) / 1e12, 2),
+                "source": os.path.relpath(tfile, ROOT)}
 
     out = d_out.cpu().numpy()
     parity = None

@@ -216,6 +216,9 @@ typedef struct pct_pruned_opts {
                         spread (median - best) take the full search; 0 = none,
                         negative = every tile (validation) */
     int shape_id;    /* tile shape (pct_tile_config.id); -1 = default */
+    int max_slabs;   /* kept offsets come from at most this many dz planes of
+                        the window: the lattice's and the best-scoring others
+                        (bounds the LDS staging per tile); 0 = any */
 } pct_pruned_opts;
 
 /* Learned-pruned fusion.  Per tile of target voxels and per atlas, the
@@ -292,6 +295,13 @@ void pct_to_hu(float* arr, size_t n);
 
 /* The whole chain of cpp:1725-1739: regularize, round, to HU. */
 int pct_postprocess(int nh, int nx, int ny, int nz, float* arr);
+
+/* Device versions of pct_regularize and pct_postprocess on a device volume,
+ * enqueued on `stream`; bit-identical results.  The NaN fill visits the NaN
+ * voxels in the reference's scan order, one at a time (cost grows with the
+ * NaN count, not the volume).  Uses the library workspace. */
+int pct_regularize_dev(int nh, int nx, int ny, int nz, float* d_arr, pct_stream_t stream);
+int pct_postprocess_dev(int nh, int nx, int ny, int nz, float* d_arr, pct_stream_t stream);
 
 /* ---- on-disk formats and inputs ---- */
 

@@ -7,7 +7,8 @@ from .api import (CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, autotune,  # noqa:
                   autotune_batch, fuse_batch, set_batch_config,
                   device_count, force_tile_config, fuse_device, fused_to_hu_raw,
                   matrixio_read, matrixio_write, multi_patch_based_segmentation_gpu,
-                  normalize_labels, normalize_labels_device, postprocess, problem, pseudo_ct,
+                  normalize_labels, normalize_labels_device, postprocess, postprocess_device,
+                  problem, pseudo_ct, regularization_step_device,
                   read_config, regularization_step, round_matrix, synth_tissue, synth_uniform,
                   tile_config, to_hu, weight_divisor)
 
@@ -16,6 +17,7 @@ __all__ = [
     "fuse_batch", "set_batch_config", "device_count",
     "force_tile_config", "fuse_device", "fused_to_hu_raw", "matrixio_read", "matrixio_write",
     "multi_patch_based_segmentation_gpu", "normalize_labels", "normalize_labels_device",
-    "postprocess", "problem", "pseudo_ct", "read_config", "regularization_step", "round_matrix",
+    "postprocess", "postprocess_device", "problem", "pseudo_ct", "read_config",
+    "regularization_step", "regularization_step_device", "round_matrix",
     "synth_tissue", "synth_uniform", "tile_config", "to_hu", "weight_divisor",
 ]

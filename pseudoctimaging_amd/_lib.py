@@ -65,7 +65,8 @@ class VolumeSet(Structure):
 
 class PrunedOpts(Structure):
     """pct_pruned_opts"""
-    _fields_ = [("topk", c_int), ("samples", c_int), ("margin", c_float), ("shape_id", c_int)]
+    _fields_ = [("topk", c_int), ("samples", c_int), ("margin", c_float), ("shape_id", c_int),
+                ("max_slabs", c_int)]
 
 
 class TrainOpts(Structure):
@@ -117,6 +118,8 @@ SIGNATURES = {
     "pct_round": (None, [FP, c_size_t]),
     "pct_to_hu": (None, [FP, c_size_t]),
     "pct_postprocess": (c_int, [c_int, c_int, c_int, c_int, FP]),
+    "pct_regularize_dev": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "pct_postprocess_dev": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
     "pct_matrixio_read": (c_int, [c_char_p, POINTER(FP), POINTER(c_int)]),
     "pct_matrixio_write": (c_int, [c_char_p, FP, c_int, c_int, c_int, c_int]),
     "pct_read_config": (c_int, [c_char_p, c_uint64, POINTER(VolumeSet)]),

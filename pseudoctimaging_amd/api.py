@@ -219,6 +219,26 @@ def postprocess(arr: np.ndarray, nh: int) -> np.ndarray:
     return arr
 
 
+def regularization_step_device(arr, nh: int, stream=None) -> None:
+    """pct_regularize_dev on a (nz, nx, ny) float32 device tensor."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream(arr.device)
+    nz, nx, ny = arr.shape
+    check(lib().pct_regularize_dev(int(nh), nx, ny, nz, _dev_ptr(arr, "arr"),
+                                   ctypes.c_void_p(stream.cuda_stream)), "pct_regularize_dev")
+
+
+def postprocess_device(arr, nh: int, stream=None) -> None:
+    """pct_postprocess_dev (NaN fill, round, HU) on a device tensor, in place."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream(arr.device)
+    nz, nx, ny = arr.shape
+    check(lib().pct_postprocess_dev(int(nh), nx, ny, nz, _dev_ptr(arr, "arr"),
+                                    ctypes.c_void_p(stream.cuda_stream)), "pct_postprocess_dev")
+
+
 def pseudo_ct(img, atlas_mri, atlas_ct_hu, nh: int, sigma: float = 1.0, beta: float = 1.0,
               choice: int = CHOICE_SM):
     """main()'s pipeline (cpp:1583-1739): normalise the atlas CT, fuse,

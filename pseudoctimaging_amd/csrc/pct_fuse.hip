@@ -795,6 +795,7 @@ extern "C" int pct_autotune(const pct_problem* pb, const float* d_img, const flo
 
 extern "C" void pct_release(void) {
     release_batch_graphs();
+    release_learned_models();
     std::lock_guard<std::mutex> lk(g_dev_mutex);
     for (auto& kv : g_dev) {
         if (kv.second.ws) {

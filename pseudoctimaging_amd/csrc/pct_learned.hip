@@ -25,8 +25,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "pct_internal.h"
@@ -47,27 +49,54 @@ constexpr int kMaxOffsets = kMaxTopk + kSamples;
 // embedding
 // ---------------------------------------------------------------------------
 
+typedef short v2s __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+
+constexpr int kEmbSeg = 256;              // voxels along y per wave job
+constexpr int kEmbRow = kEmbSeg + 2;      // staged row length (one-voxel halo)
+constexpr int kEmbRows = 10;              // 3x3 neighbour rows + one zero row
+constexpr int kEmbWaves = 4;
+
 struct EmbedArgs {
     const float* vol;               // [nvol][nz][nx][ny]
     unsigned short* emb;            // [nvol][nz][nx][ny][16] BF16
     const v4s* frag;                // [28][64] weight fragments
     const float* bias;              // b1[64], b2[64], b3[16]
     int nx, ny, nz, nvol;
+    int nseg;                       // ceil(ny / kEmbSeg)
+    int jobs;                       // nvol * nz * nx * nseg row segments
     float mu, inv_sd;
-    long long chunks;               // nvol * nz * nx * ceil(ny / 16)
 };
 
-__device__ __forceinline__ v4s relu_bf16(v4f h) {
-    v4s r;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) r[j] = (short)f2bf(fmaxf(h[j], 0.f));
-    return r;
+// Four FP32 values to BF16 (round to nearest even, v_cvt_pk_bf16_f32).
+__device__ __forceinline__ v4s to_bf16(float a, float b, float c, float d) {
+    const v2s lo = __builtin_bit_cast(v2s, __builtin_convertvector((f2){a, b}, bf2));
+    const v2s hi = __builtin_bit_cast(v2s, __builtin_convertvector((f2){c, d}, bf2));
+    return (v4s){lo[0], lo[1], hi[0], hi[1]};
 }
 
-// One wave embeds 16 consecutive voxels along y per iteration.  Lane l:
-// voxel n = l % 16 of the chunk, feature/hidden group g = l / 16.
-__global__ __launch_bounds__(256) void embed_kernel(EmbedArgs a) {
-    const int lane = threadIdx.x & 63;
+// ReLU then BF16 of four accumulators.  Rounding commutes with the clamp,
+// and a BF16 word read as int16 is negative exactly when the value is, so
+// the clamp is one packed 16-bit integer max per pair.
+__device__ __forceinline__ v4s relu_bf16(v4f h) {
+    const v4s r = to_bf16(h[0], h[1], h[2], h[3]);
+    const v2s zero = {0, 0};
+    const v2s lo = __builtin_elementwise_max((v2s){r[0], r[1]}, zero);
+    const v2s hi = __builtin_elementwise_max((v2s){r[2], r[3]}, zero);
+    return (v4s){lo[0], lo[1], hi[0], hi[1]};
+}
+
+// One wave embeds one row segment (volume, z, x, up to kEmbSeg voxels along
+// y).  It first stages the normalised values g of the 3x3 neighbouring rows
+// (z-1..z+1, x-1..x+1) with a one-voxel y halo in LDS, zero outside the
+// volume, plus a zero row; feature k of the voxel at local column yl is then
+// sc_k * (row[oa_k + yl] - row[ob_k + yl]) with per-lane constants.  The
+// MFMA chain handles 16 voxels per step: lane l holds voxel n = l % 16 and
+// feature / hidden group g = l / 16.
+__global__ __launch_bounds__(kEmbWaves * 64) void embed_kernel(EmbedArgs a) {
+    __shared__ float rows[kEmbWaves][kEmbRows * kEmbRow];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, n = lane & 15;
     v4s w1[4][2], w2[4][4], w3[4];
 #pragma unroll
@@ -91,59 +120,109 @@ __global__ __launch_bounds__(256) void embed_kernel(EmbedArgs a) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) b3[j] = a.bias[128 + 4 * g + j];
 
-    const int ychunks = (a.ny + 15) / 16;
-    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
-    const long long vsize = (long long)a.nx * a.ny * a.nz;
-    for (long long c = wave; c < a.chunks; c += nwaves) {
-        const int yc = (int)(c % ychunks);
-        long long row = c / ychunks;
-        const int x = (int)(row % a.nx);
-        row /= a.nx;
-        const int z = (int)(row % a.nz);
-        const int vi = (int)(row / a.nz);
-        const int y = yc * 16 + n;
-        const float* __restrict__ v = a.vol + vi * vsize;
-        v4s xb[2];
+    // gather plan of the lane's 8 features k = 16 s + 4 g + j
+    constexpr int kZero = 9 * kEmbRow;
+    int oa[2][4], ob[2][4];
+    float sc[2][4];
 #pragma unroll
-        for (int s = 0; s < 2; ++s)
+    for (int s = 0; s < 2; ++s)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                xb[s][j] = (short)f2bf(
-                    feature(v, a.nx, a.ny, a.nz, z, x, y, 16 * s + 4 * g + j, a.mu, a.inv_sd));
-        v4f h1[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            v4f acc = b1[t];
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[t][0], xb[0], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[t][1], xb[1], acc, 0, 0, 0);
-            h1[t] = acc;
+        for (int j = 0; j < 4; ++j) {
+            const int k = 16 * s + 4 * g + j;
+            int o1 = kZero, o2 = kZero;
+            float c = 0.f;
+            if (k < 27) {                 // row 3 pz + px, column offset py
+                o1 = (k / 3) * kEmbRow + k % 3;
+                c = 1.f;
+            } else if (k == 27) {         // z: rows (2,1) and (0,1)
+                o1 = 7 * kEmbRow + 1;
+                o2 = 1 * kEmbRow + 1;
+                c = 0.5f;
+            } else if (k == 28) {         // x: rows (1,2) and (1,0)
+                o1 = 5 * kEmbRow + 1;
+                o2 = 3 * kEmbRow + 1;
+                c = 0.5f;
+            } else if (k == 29) {         // y: row (1,1), columns +1 and -1
+                o1 = 4 * kEmbRow + 2;
+                o2 = 4 * kEmbRow + 0;
+                c = 0.5f;
+            }
+            oa[s][j] = o1;
+            ob[s][j] = o2;
+            sc[s][j] = c;
         }
-        v4s h1b[4];
+
+    float* __restrict__ R = rows[wv];
+    const long long vsize = (long long)a.nx * a.ny * a.nz;
+    for (int base = blockIdx.x * kEmbWaves; base < a.jobs; base += gridDim.x * kEmbWaves) {
+        const int job = base + wv;
+        const bool active = job < a.jobs;
+        int seg = 0, x = 0, z = 0, vi = 0;
+        if (active) {
+            seg = job % a.nseg;
+            int r = job / a.nseg;
+            x = r % a.nx;
+            r /= a.nx;
+            z = r % a.nz;
+            vi = r / a.nz;
+        }
+        const int y0 = seg * kEmbSeg;
+        const int len = min(kEmbSeg, a.ny - y0);
+        __syncthreads();   // the previous job's reads of R are done
+        if (active) {
+            const float* __restrict__ v = a.vol + vi * vsize;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) h1b[t] = relu_bf16(h1[t]);
-        v4f h2[4];
+            for (int rr = 0; rr < kEmbRows; ++rr) {
+                const int zz = z + rr / 3 - 1, xx = x + rr % 3 - 1;
+                const bool rok = rr < 9 && zz >= 0 && zz < a.nz && xx >= 0 && xx < a.nx;
+                const float* __restrict__ src = v + ((long long)zz * a.nx + xx) * a.ny;
+                for (int c = lane; c < kEmbRow; c += 64) {
+                    const int yy = y0 - 1 + c;
+                    float t = 0.f;
+                    if (rok && c < len + 2 && yy >= 0 && yy < a.ny) t = (src[yy] - a.mu) * a.inv_sd;
+                    R[rr * kEmbRow + c] = t;
+                }
+            }
+        }
+        __syncthreads();
+        if (!active) continue;
+        unsigned short* __restrict__ dst0 =
+            a.emb + (vi * vsize + ((long long)z * a.nx + x) * a.ny + y0) * 16 + 4 * g;
+        for (int c0 = 0; c0 < len; c0 += 16) {
+            const int yl = c0 + n;
+            float f[2][4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            v4f acc = b2[t];
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    f[s][j] = sc[s][j] * (R[oa[s][j] + yl] - R[ob[s][j] + yl]);
+            const v4s xb0 = to_bf16(f[0][0], f[0][1], f[0][2], f[0][3]);
+            const v4s xb1 = to_bf16(f[1][0], f[1][1], f[1][2], f[1][3]);
+            v4s h1b[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                v4f acc = b1[t];
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[t][0], xb0, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[t][1], xb1, acc, 0, 0, 0);
+                h1b[t] = relu_bf16(acc);
+            }
+            v4s h2b[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                v4f acc = b2[t];
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2[t][s], h1b[s], acc, 0, 0, 0);
+                h2b[t] = relu_bf16(acc);
+            }
+            v4f e = b3;
 #pragma unroll
             for (int s = 0; s < 4; ++s)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2[t][s], h1b[s], acc, 0, 0, 0);
-            h2[t] = acc;
-        }
-        v4s h2b[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) h2b[t] = relu_bf16(h2[t]);
-        v4f e = b3;
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-            e = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3[s], h2b[s], e, 0, 0, 0);
-        if (y < a.ny) {
-            const unsigned lo = (unsigned)f2bf(e[0]) | ((unsigned)f2bf(e[1]) << 16);
-            const unsigned hi = (unsigned)f2bf(e[2]) | ((unsigned)f2bf(e[3]) << 16);
-            uint2* dst = reinterpret_cast<uint2*>(
-                a.emb + ((vi * vsize + ((long long)z * a.nx + x) * a.ny + y) * 16 + 4 * g));
-            *dst = make_uint2(lo, hi);
+                e = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3[s], h2b[s], e, 0, 0, 0);
+            if (yl < len) {
+                const v4s eb = to_bf16(e[0], e[1], e[2], e[3]);
+                *reinterpret_cast<uint2*>(dst0 + (long long)yl * 16) = __builtin_bit_cast(uint2, eb);
+            }
         }
     }
 }
@@ -177,21 +256,65 @@ static void make_fragments(const float* p, std::vector<unsigned short>& frag,
     for (int i = 0; i < 16; ++i) bias[128 + i] = p[kOffB3 + i];
 }
 
-// Uploads the fragments and biases to `dst` (kFrags*64*8 + 144*4 bytes).
-static int upload_model(const float* params, void* dst, hipStream_t stream) {
-    std::vector<unsigned short> frag;
-    std::vector<float> bias;
-    make_fragments(params, frag, bias);
-    char* d = static_cast<char*>(dst);
-    if (hipMemcpyAsync(d, frag.data(), frag.size() * 2, hipMemcpyHostToDevice, stream) !=
-            hipSuccess ||
-        hipMemcpyAsync(d + frag.size() * 2, bias.data(), bias.size() * 4, hipMemcpyHostToDevice,
-                       stream) != hipSuccess)
-        return PCT_ERR_HIP;
-    // the host vectors die here: make the copies complete first
-    return hipStreamSynchronize(stream) == hipSuccess ? PCT_OK : PCT_ERR_HIP;
-}
 constexpr size_t kModelBytes = kFrags * 64 * 8 + 144 * 4;
+
+// Per-device copy of the weight fragments and biases, uploaded again only
+// when the parameter vector changes.  The upload is ordered on the caller's
+// stream; calls that overlap on one device must share a stream (as for the
+// workspace).
+struct ModelCache {
+    std::mutex mu;
+    void* d_model[64] = {};
+    std::vector<float> last[64];
+};
+static ModelCache g_model;
+
+void release_learned_models() {
+    std::lock_guard<std::mutex> lk(g_model.mu);
+    for (int dev = 0; dev < 64; ++dev) {
+        if (!g_model.d_model[dev]) continue;
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(dev);
+        (void)hipDeviceSynchronize();
+        (void)hipFree(g_model.d_model[dev]);
+        (void)hipSetDevice(cur);
+        g_model.d_model[dev] = nullptr;
+        g_model.last[dev].clear();
+    }
+}
+
+static int device_model(int dev, const float* params, hipStream_t stream, const void** out) {
+    if (dev < 0 || dev >= 64) return PCT_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> lk(g_model.mu);
+    if (!g_model.d_model[dev]) {
+        if (hipMalloc(&g_model.d_model[dev], kModelBytes) != hipSuccess) {
+            (void)hipGetLastError();
+            g_model.d_model[dev] = nullptr;
+            return PCT_ERR_NO_MEMORY;
+        }
+        g_model.last[dev].clear();
+    }
+    std::vector<float>& last = g_model.last[dev];
+    if (last.size() != PCT_MLP_NPARAMS ||
+        std::memcmp(last.data(), params, sizeof(float) * PCT_MLP_NPARAMS) != 0) {
+        std::vector<unsigned short> frag;
+        std::vector<float> bias;
+        make_fragments(params, frag, bias);
+        char* d = static_cast<char*>(g_model.d_model[dev]);
+        if (hipMemcpyAsync(d, frag.data(), frag.size() * 2, hipMemcpyHostToDevice, stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(d + frag.size() * 2, bias.data(), bias.size() * 4,
+                           hipMemcpyHostToDevice, stream) != hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess) {   // the host vectors die here
+            last.clear();
+            return PCT_ERR_HIP;
+        }
+        last.assign(params, params + PCT_MLP_NPARAMS);
+    }
+    *out = g_model.d_model[dev];
+    return PCT_OK;
+}
 
 static int launch_embed(const float* params, const void* d_model, const float* vol, int nx,
                         int ny, int nz, int nvol, unsigned short* emb, hipStream_t stream) {
@@ -204,12 +327,15 @@ static int launch_embed(const float* params, const void* d_model, const float* v
     a.ny = ny;
     a.nz = nz;
     a.nvol = nvol;
+    a.nseg = (ny + kEmbSeg - 1) / kEmbSeg;
+    const long long jobs = (long long)nvol * nz * nx * a.nseg;
+    if (jobs >= INT_MAX) return PCT_ERR_TOO_LARGE;
+    a.jobs = (int)jobs;
     a.mu = params[kOffMu];
     a.inv_sd = params[kOffSd];
-    a.chunks = (long long)nvol * nz * nx * ((ny + 15) / 16);
-    const long long waves = std::min<long long>(a.chunks, 256LL * 32);
-    const unsigned blocks = (unsigned)((waves + 3) / 4);
-    hipLaunchKernelGGL(embed_kernel, dim3(blocks), dim3(256), 0, stream, a);
+    const long long blocks = std::min<long long>((jobs + kEmbWaves - 1) / kEmbWaves, 256LL * 16);
+    if (blocks > 0)
+        hipLaunchKernelGGL(embed_kernel, dim3((unsigned)blocks), dim3(kEmbWaves * 64), 0, stream, a);
     return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;
 }
 
@@ -246,6 +372,61 @@ __global__ void box_pass(const float* __restrict__ in, float* __restrict__ out, 
     }
 }
 
+// The same sums along y: one wave per row, the row staged in LDS with hw
+// zeros on each side, outputs summed from LDS in the order of box_pass.
+__global__ __launch_bounds__(256) void box_y(const float* __restrict__ in,
+                                             float* __restrict__ out, int ny, long long rows,
+                                             int hw) {
+    extern __shared__ float sh[];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int rl = ny + 2 * hw;
+    float* R = sh + wv * rl;
+    const long long row = (long long)blockIdx.x * 4 + wv;
+    const bool active = row < rows;
+    if (active) {
+        const float* __restrict__ src = in + row * ny;
+        for (int c = lane; c < rl; c += 64) {
+            const int y = c - hw;
+            R[c] = (y >= 0 && y < ny) ? src[y] : 0.f;
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    float* __restrict__ dst = out + row * ny;
+    for (int y = lane; y < ny; y += 64) {
+        float acc = 0.f;
+        for (int k = 0; k <= 2 * hw; ++k) acc += R[y + k];
+        dst[y] = acc;
+    }
+}
+
+// The same sums along x (axis 1) or z (axis 0) as a running sum in double:
+// one thread per line, the lanes of a wave on consecutive y (coalesced), two
+// reads per output instead of 2 hw + 1.
+__global__ void box_line(const float* __restrict__ in, float* __restrict__ out, int nx, int ny,
+                         int nz, int nvol, int hw, int axis) {
+    const int other = axis == 1 ? nz : nx;       // the line's remaining coordinate
+    const long long lines = (long long)nvol * other * ny;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= lines) return;
+    const int y = (int)(t % ny);
+    const long long rr = t / ny;
+    const int o = (int)(rr % other);
+    const long long vi = rr / other;
+    const int len = axis == 1 ? nx : nz;
+    const long long stride = axis == 1 ? (long long)ny : (long long)nx * ny;
+    const long long base = vi * nx * ny * (long long)nz +
+                           (axis == 1 ? (long long)o * nx * ny : (long long)o * ny) + y;
+    double acc = 0.0;
+    for (int k = 0; k <= hw && k < len; ++k) acc += in[base + k * stride];
+    for (int i = 0; i < len; ++i) {
+        out[base + i * stride] = (float)acc;
+        const int add = i + hw + 1, sub = i - hw;
+        if (add < len) acc += in[base + add * stride];
+        if (sub >= 0) acc -= in[base + sub * stride];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // candidate selection
 // ---------------------------------------------------------------------------
@@ -253,7 +434,7 @@ __global__ void box_pass(const float* __restrict__ in, float* __restrict__ out, 
 struct SelectArgs {
     const unsigned short* embT;   // [N][16]
     const unsigned short* embA;   // [n][N][16]
-    int* sel;                     // [items][topk] kept offset indices, ascending
+    int* sel;                     // [items][topk] kept offset indices, ascending, kNoOffset after
     unsigned char* flag;          // [items] 1 = full search
     long long N;
     int nx, ny, nz, hw, r;
@@ -261,14 +442,21 @@ struct SelectArgs {
     int oy, wx, bz;               // tile extents in y, x, z
     int topk;
     float margin;
+    int max_slabs;                // distinct dz slabs the kept offsets may use; 0 = all
+    int n_lattice_dz;
+    int lattice_dz[3];            // dz of the lattice samples (always allowed)
 };
 
 constexpr int kSelThreads = 256;
-constexpr int kSelPerThread = 16;   // 4096 sort slots (NH = 15: 3375 offsets)
+constexpr int kSelPerThread = 16;   // 4096 offsets (NH = 15: 3375)
+constexpr int kNoOffset = 1 << 30;
 
 // One workgroup per item = (atlas, tile), item = atlas * n_tiles + tile.
 __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
     extern __shared__ float sh[];
+    __shared__ float dzmin[16];
+    __shared__ int allowed[16];       // allowed dz indices, ascending
+    __shared__ int n_allowed;
     const int item = blockIdx.x;
     const int at = item / a.n_tiles;
     int t = item - at * a.n_tiles;
@@ -276,8 +464,8 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
     t /= a.tiles_y;
     const int tx = t % a.tiles_x;
     const int tz = t / a.tiles_x;
-    const int r = a.r, hw = a.hw, nw = 2 * hw + 1;
-    const int ncand = nw * nw * nw;
+    const int r = a.r, hw = a.hw, nw = 2 * hw + 1, nw2 = nw * nw;
+    const int ncand = nw2 * nw;
     // probe cluster: 4 (y) x 2 (x) x 2 (z) voxels at the centre of the
     // tile's part of the interior
     const int y0 = r + ty * a.oy, x0 = r + tx * a.wx, z0 = r + tz * a.bz;
@@ -286,10 +474,11 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
     const int py0 = max(r, min((y0 + ye) / 2 - 2, a.ny - r - 4));
     const int px0 = max(r, min((x0 + xe) / 2 - 1, a.nx - r - 2));
     const int pz0 = max(r, min((z0 + ze) / 2 - 1, a.nz - r - 2));
-    // LDS: probe target embeddings (16 x 16 floats) and the atlas embeddings
-    // of the cluster's window (BF16 pairs); afterwards the sort keys/values.
-    // Coordinates are clamped into the volume (a thin interior can leave the
-    // cluster's window partly outside).
+    // LDS: probe target embeddings (16 x 16 floats), then the atlas
+    // embeddings of the cluster's window as 8 planes of BF16 pairs (plane k2
+    // holds dimensions 2 k2, 2 k2 + 1 of every window voxel: lanes scoring
+    // neighbouring offsets read neighbouring words).  Coordinates are clamped
+    // into the volume (a thin interior can leave the window partly outside).
     float* sT = sh;
     unsigned* sA = reinterpret_cast<unsigned*>(sh + 256);
     const int WY = 4 + 2 * hw, WXX = 2 + 2 * hw, WZ = 2 + 2 * hw;
@@ -309,28 +498,26 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
         const int yy = min(max(py0 - hw + wy, 0), a.ny - 1);
         const int xx = min(max(px0 - hw + wx, 0), a.nx - 1);
         const int zz = min(max(pz0 - hw + wz, 0), a.nz - 1);
-        sA[i] = eA[(zz * plane + (long long)xx * a.ny + yy) * 8 + k2];
+        sA[k2 * nwin + w] = eA[(zz * plane + (long long)xx * a.ny + yy) * 8 + k2];
     }
     __syncthreads();
     // scores: thread handles offsets c = threadIdx.x + q * kSelThreads
-    int p2 = 1;
-    while (p2 < ncand) p2 <<= 1;
     float sc[kSelPerThread];
 #pragma unroll
     for (int q = 0; q < kSelPerThread; ++q) {
         const int c = threadIdx.x + q * kSelThreads;
         float score = __builtin_huge_valf();
         if (c < ncand) {
-            const int dz = c / (nw * nw) - hw, dx = (c / nw) % nw - hw, dy = c % nw - hw;
+            const int dz = c / nw2 - hw, dx = (c / nw) % nw - hw, dy = c % nw - hw;
             score = 0.f;
             for (int p = 0; p < 16; ++p) {
                 const int wy = (p & 3) + hw + dy, wx = ((p >> 2) & 1) + hw + dx,
                           wz = (p >> 3) + hw + dz;
-                const unsigned* ea = sA + ((wz * WXX + wx) * WY + wy) * 8;
+                const unsigned* ea = sA + (wz * WXX + wx) * WY + wy;
                 const float* et = sT + p * 16;
 #pragma unroll
                 for (int k2 = 0; k2 < 8; ++k2) {
-                    const unsigned u = ea[k2];
+                    const unsigned u = ea[k2 * nwin];
                     const float d0 = et[2 * k2] - __uint_as_float(u << 16);
                     const float d1 = et[2 * k2 + 1] - __uint_as_float(u & 0xffff0000u);
                     score = fmaf(d0, d0, score);
@@ -341,55 +528,106 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
         sc[q] = score;
     }
     __syncthreads();   // the window is dead from here on
-    float* key = sh;
-    int* val = reinterpret_cast<int*>(sh + p2);
+    float* key = sh;   // [ncand] scores by offset index
 #pragma unroll
     for (int q = 0; q < kSelPerThread; ++q) {
         const int c = threadIdx.x + q * kSelThreads;
-        if (c < p2) {
-            key[c] = sc[q];
-            val[c] = c;
+        if (c < ncand) key[c] = sc[q];
+    }
+    __syncthreads();
+    // best score of every dz slab, then the allowed slabs: the lattice's and
+    // the best others up to max_slabs (ties to the smaller dz)
+    if (threadIdx.x < nw) {
+        float m = __builtin_huge_valf();
+        for (int c = threadIdx.x * nw2; c < (threadIdx.x + 1) * nw2; ++c) m = fminf(m, key[c]);
+        dzmin[threadIdx.x] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bool ok[16];
+        for (int d = 0; d < nw; ++d) ok[d] = a.max_slabs <= 0 || a.max_slabs >= nw;
+        int cnt = 0;
+        for (int d = 0; d < nw; ++d) cnt += ok[d];
+        for (int l = 0; l < a.n_lattice_dz && cnt < nw; ++l) {
+            const int d = a.lattice_dz[l] + hw;
+            if (!ok[d]) {
+                ok[d] = true;
+                ++cnt;
+            }
+        }
+        while (cnt < a.max_slabs && cnt < nw) {
+            int best = -1;
+            for (int d = 0; d < nw; ++d)
+                if (!ok[d] && (best < 0 || dzmin[d] < dzmin[best])) best = d;
+            ok[best] = true;
+            ++cnt;
+        }
+        int m = 0;
+        for (int d = 0; d < nw; ++d)
+            if (ok[d]) allowed[m++] = d;
+        n_allowed = m;
+    }
+    __syncthreads();
+    // candidates of the allowed slabs, then bitonic sort ascending by
+    // (score, offset index)
+    const int m = n_allowed * nw2;
+    int p2 = 1;
+    while (p2 < m) p2 <<= 1;
+    float* key2 = sh + ((ncand + 63) & ~63);
+    int* val2 = reinterpret_cast<int*>(key2 + p2);
+    for (int i = threadIdx.x; i < p2; i += kSelThreads) {
+        if (i < m) {
+            const int c = allowed[i / nw2] * nw2 + i % nw2;
+            key2[i] = key[c];
+            val2[i] = c;
+        } else {
+            key2[i] = __builtin_huge_valf();
+            val2[i] = kNoOffset + i;
         }
     }
     __syncthreads();
-    // bitonic sort ascending by (score, offset index)
     for (int size = 2; size <= p2; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
             for (int i = threadIdx.x; i < p2 / 2; i += kSelThreads) {
                 const int lo = 2 * i - (i & (stride - 1));
                 const int hi = lo + stride;
                 const bool up = ((lo & size) == 0);
-                const float kl = key[lo], kh = key[hi];
-                const int vl = val[lo], vh = val[hi];
+                const float kl = key2[lo], kh = key2[hi];
+                const int vl = val2[lo], vh = val2[hi];
                 const bool gt = kl > kh || (kl == kh && vl > vh);
                 if (gt == up) {
-                    key[lo] = kh;
-                    key[hi] = kl;
-                    val[lo] = vh;
-                    val[hi] = vl;
+                    key2[lo] = kh;
+                    key2[hi] = kl;
+                    val2[lo] = vh;
+                    val2[hi] = vl;
                 }
             }
             __syncthreads();
         }
     }
-    const int k = min(a.topk, ncand);
+    const int k = min(a.topk, m);
     if (threadIdx.x == 0) {
         // a small gap between the last kept and the first dropped score,
         // relative to the spread of the scores, flags the item
         int f = a.margin < 0.f ? 1 : 0;
-        if (!f && k < ncand) {
-            const float spread = key[ncand / 2] - key[0];
-            const float gap = key[k] - key[k - 1];
+        if (!f && k < m) {
+            const float spread = key2[m / 2] - key2[0];
+            const float gap = key2[k] - key2[k - 1];
             f = (spread > 0.f && gap < a.margin * spread) ? 1 : 0;
         }
         a.flag[item] = (unsigned char)f;
     }
     // kept offsets in ascending index order (dz-major), by rank counting
-    for (int i = threadIdx.x; i < k; i += kSelThreads) {
-        const int v = val[i];
-        int rank = 0;
-        for (int j = 0; j < k; ++j) rank += val[j] < v;
-        a.sel[(long long)item * a.topk + rank] = v;
+    int* out = a.sel + (long long)item * a.topk;
+    for (int i = threadIdx.x; i < a.topk; i += kSelThreads) {
+        if (i < k) {
+            const int v = val2[i];
+            int rank = 0;
+            for (int j = 0; j < k; ++j) rank += val2[j] < v;
+            out[rank] = v;
+        } else {
+            out[i] = kNoOffset;
+        }
     }
 }
 
@@ -490,9 +728,10 @@ __global__ __launch_bounds__(NW * 64) void pruned_kernel(PrunedArgs pa) {
         // a sample that is also kept counts once, as kept
         const int* ks = pa.sel + (long long)item * pa.topk;
         int i = 0, j = 0, m = 0, ns = 0;
-        while (i < pa.topk || j < pa.n_samples) {
-            const int a = i < pa.topk ? ks[i] : 1 << 30;
-            const int b = j < pa.n_samples ? pa.samples[j] : 1 << 30;
+        for (;;) {
+            const int a = i < pa.topk ? ks[i] : kNoOffset;
+            const int b = j < pa.n_samples ? pa.samples[j] : kNoOffset;
+            if (a >= kNoOffset && b >= kNoOffset) break;
             if (a < b) {
                 offs[m] = a; is_sample[m] = 0; ++m; ++i;
             } else if (b < a) {
@@ -641,12 +880,11 @@ extern "C" int pct_embed_dev(const float* params, const float* d_vol, int nx, in
     int rc = check_device(&dev);
     if (rc) return rc;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    void* ws = nullptr;
-    rc = ensure_workspace(dev, kModelBytes + 256, &ws);
+    const void* model = nullptr;
+    rc = device_model(dev, params, s, &model);
     if (rc) return rc;
-    rc = upload_model(params, ws, s);
-    if (rc) return rc;
-    return launch_embed(params, ws, d_vol, nx, ny, nz, 1, reinterpret_cast<unsigned short*>(d_emb), s);
+    return launch_embed(params, model, d_vol, nx, ny, nz, 1,
+                        reinterpret_cast<unsigned short*>(d_emb), s);
 }
 
 extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
@@ -678,7 +916,6 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     // workspace layout
     auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
     size_t off = 0;
-    const size_t o_model = off; off = align(off + kModelBytes);
     const size_t o_part = off; off = align(off + sizeof(float) * 2 * (size_t)n * N);
     const size_t o_embT = off; off = align(off + 32 * (size_t)N);
     const size_t o_embA = off; off = align(off + 32 * (size_t)N * n);
@@ -690,30 +927,55 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     rc = ensure_workspace(dev, off, &ws);
     if (rc) return rc;
     char* w = static_cast<char*>(ws);
-    rc = upload_model(params, w + o_model, s);
+    const void* model = nullptr;
+    rc = device_model(dev, params, s, &model);
     if (rc) return rc;
 
     // 1. embeddings of the target and of every atlas MRI
     unsigned short* embT = reinterpret_cast<unsigned short*>(w + o_embT);
     unsigned short* embA = reinterpret_cast<unsigned short*>(w + o_embA);
-    rc = launch_embed(params, w + o_model, d_img, pb->nx, pb->ny, pb->nz, 1, embT, s);
-    if (!rc) rc = launch_embed(params, w + o_model, d_atlas, pb->nx, pb->ny, pb->nz, n, embA, s);
+    rc = launch_embed(params, model, d_img, pb->nx, pb->ny, pb->nz, 1, embT, s);
+    if (!rc) rc = launch_embed(params, model, d_atlas, pb->nx, pb->ny, pb->nz, n, embA, s);
     if (rc) return rc;
 
     // 2. window sums of every atlas CT: y into tmp, x into boxL, z into tmp
     float* boxL = reinterpret_cast<float*>(w + o_boxL);
     float* tmp = reinterpret_cast<float*>(w + o_tmp);
-    const unsigned bb = (unsigned)std::min<long long>((N * n + 255) / 256, 65536);
-    hipLaunchKernelGGL(box_pass, dim3(bb), dim3(256), 0, s, d_label, tmp, pb->nx, pb->ny, pb->nz,
-                       n, hw, 2);
-    hipLaunchKernelGGL(box_pass, dim3(bb), dim3(256), 0, s, tmp, boxL, pb->nx, pb->ny, pb->nz, n,
-                       hw, 1);
-    hipLaunchKernelGGL(box_pass, dim3(bb), dim3(256), 0, s, boxL, tmp, pb->nx, pb->ny, pb->nz, n,
-                       hw, 0);
+    const long long rows_y = (long long)n * pb->nz * pb->nx;
+    const size_t ylds = sizeof(float) * 4 * ((size_t)pb->ny + 2 * hw);
+    if (ylds <= 64 * 1024) {
+        hipLaunchKernelGGL(box_y, dim3((unsigned)((rows_y + 3) / 4)), dim3(256), ylds, s,
+                           d_label, tmp, pb->ny, rows_y, hw);
+    } else {
+        const unsigned bb = (unsigned)std::min<long long>((N * n + 255) / 256, 65536);
+        hipLaunchKernelGGL(box_pass, dim3(bb), dim3(256), 0, s, d_label, tmp, pb->nx, pb->ny,
+                           pb->nz, n, hw, 2);
+    }
+    const long long lx = (long long)n * pb->nz * pb->ny, lz = (long long)n * pb->nx * pb->ny;
+    hipLaunchKernelGGL(box_line, dim3((unsigned)((lx + 255) / 256)), dim3(256), 0, s, tmp, boxL,
+                       pb->nx, pb->ny, pb->nz, n, hw, 1);
+    hipLaunchKernelGGL(box_line, dim3((unsigned)((lz + 255) / 256)), dim3(256), 0, s, boxL, tmp,
+                       pb->nx, pb->ny, pb->nz, n, hw, 0);
     if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
     std::swap(boxL, tmp);
 
-    // 3. selection
+    // 3. selection; the lattice samples are 3x3x3 offsets at 0 and
+    // +-ceil(2 hw / 3) (all 27 offsets for NH = 3)
+    const int nw = 2 * hw + 1;
+    int samples[kSamples] = {0};
+    int n_samples = 0;
+    {
+        const int q = std::max(1, (2 * hw + 2) / 3);
+        const int lat[3] = {-q, 0, q};
+        const int want = std::min(std::max(opts->samples, 0), kSamples);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int c = 0; c < 3; ++c)
+                    if (n_samples < want)
+                        samples[n_samples++] =
+                            ((lat[a] + hw) * nw + (lat[b] + hw)) * nw + (lat[c] + hw);
+        std::sort(samples, samples + n_samples);
+    }
     SelectArgs sa;
     sa.embT = embT;
     sa.embA = embA;
@@ -733,11 +995,20 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     sa.bz = sh.bz;
     sa.topk = topk;
     sa.margin = opts->margin;
+    sa.max_slabs = opts->max_slabs;
+    sa.n_lattice_dz = 0;
+    for (int i = 0; i < n_samples; ++i) {
+        const int dz = samples[i] / (nw * nw) - hw;
+        bool seen = false;
+        for (int l = 0; l < sa.n_lattice_dz; ++l) seen |= sa.lattice_dz[l] == dz;
+        if (!seen) sa.lattice_dz[sa.n_lattice_dz++] = dz;
+    }
+    if (ncand > kSelThreads * kSelPerThread) return PCT_ERR_UNSUPPORTED;
     int p2 = 1;
     while (p2 < ncand) p2 <<= 1;
-    if (p2 > kSelThreads * kSelPerThread) return PCT_ERR_UNSUPPORTED;
     const int win = (4 + 2 * hw) * (2 + 2 * hw) * (2 + 2 * hw);
-    const size_t sel_lds = std::max(256 * 4 + (size_t)win * 32, (size_t)p2 * 8);
+    const size_t sel_lds = std::max(256 * 4 + (size_t)win * 32,
+                                    (size_t)((ncand + 63) & ~63) * 4 + (size_t)p2 * 8);
     if (sel_lds > (size_t)kLdsPerCu) return PCT_ERR_UNSUPPORTED;
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(select_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sel_lds);
@@ -760,22 +1031,8 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     pa.boxL = boxL;
     pa.topk = topk;
     pa.ncand = ncand;
-    {
-        // 3x3x3 lattice at 0 and +-ceil(2 hw / 3)
-        const int q = std::max(1, (2 * hw + 2) / 3);
-        const int lat[3] = {-q, 0, q};
-        int ns = 0;
-        for (int a = 0; a < 3 && opts->samples > 0; ++a)
-            for (int b = 0; b < 3; ++b)
-                for (int c = 0; c < 3; ++c) {
-                    const int nw = 2 * hw + 1;
-                    if (ns < std::min(opts->samples, kSamples))
-                        pa.samples[ns++] =
-                            ((lat[a] + hw) * nw + (lat[b] + hw)) * nw + (lat[c] + hw);
-                }
-        std::sort(pa.samples, pa.samples + ns);
-        pa.n_samples = ns;
-    }
+    std::memcpy(pa.samples, samples, sizeof(samples));
+    pa.n_samples = n_samples;
     PrunedFn fn = pe->fn[hw - 1];
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                               hipFuncAttributeMaxDynamicSharedMemorySize, g.lds_bytes);

@@ -1,0 +1,198 @@
+// Post-fusion chain on the GPU: NaN fill by the median of the 3x3x3
+// neighbourhood, rounding, and the HU rescale, with the reference's order
+// of operations (src/CPU/NewPatchedBrainPureC.cpp:1020-1106 Median and
+// RegularizationStep, 1221-1227 RoundMatrix, 1725-1739 the HU rescale).
+//
+// RegularizationStep sweeps the interior in z, x, y order and overwrites each
+// NaN voxel in place, so a voxel sees the values filled earlier in the same
+// sweep.  Only NaN voxels ever change, so the sweep is reproduced exactly by
+// (1) listing the interior NaN voxels in scan order (one workgroup, ordered
+// prefix sums) and (2) filling them in list order with one wave (the 27
+// neighbours on 27 lanes, the median by stable ranks across lanes); voxels
+// whose median is still NaN form the next sweep's list.  Sweeps stop when no
+// NaN is left or a sweep fills nothing.  Rounding and the rescale are
+// elementwise.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+
+#include "pct_internal.h"
+#include "pct_tiled.h"
+
+namespace pct {
+
+// Interior NaN voxels of `arr` in scan order (z, x, y; y fastest) into
+// list[0..*count).  One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void nan_list_kernel(const float* __restrict__ arr, int nx,
+                                                        int ny, int nz, int lo,
+                                                        int* __restrict__ list,
+                                                        int* __restrict__ count) {
+    __shared__ int wave_cnt[16];
+    __shared__ int base;
+    if (threadIdx.x == 0) base = 0;
+    __syncthreads();
+    const int iz = nz - 2 * lo, ix = nx - 2 * lo, iy = ny - 2 * lo;
+    const long long total = (iz > 0 && ix > 0 && iy > 0) ? (long long)iz * ix * iy : 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (long long c0 = 0; c0 < total; c0 += 1024) {
+        const long long t = c0 + threadIdx.x;
+        bool isn = false;
+        int id = 0;
+        if (t < total) {
+            const int y = lo + (int)(t % iy);
+            const int x = lo + (int)((t / iy) % ix);
+            const int z = lo + (int)(t / ((long long)iy * ix));
+            id = (z * nx + x) * ny + y;
+            isn = isnan(arr[id]);
+        }
+        const unsigned long long bal = __ballot(isn);
+        const int before = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wave_cnt[w] = __popcll(bal);
+        __syncthreads();
+        int off = base;
+        for (int j = 0; j < w; ++j) off += wave_cnt[j];
+        if (isn) list[off + before] = id;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int j = 0; j < 16; ++j) tot += wave_cnt[j];
+            base += tot;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = base;
+}
+
+// Value of the valid lane whose stable rank is `pos` (all lanes receive it).
+__device__ __forceinline__ float value_at_rank(float v, bool ok, int rank, int pos) {
+    const unsigned long long sel = __ballot(ok && rank == pos);
+    const int src = __ffsll((long long)sel) - 1;
+    return __shfl(v, src);
+}
+
+// The sweeps of RegularizationStep over the listed voxels: one wave.
+// Neighbour values are read with agent-scope atomic loads and each fill is
+// stored the same way and fenced, so a later voxel of the same sweep sees it.
+__global__ __launch_bounds__(64) void nan_fill_kernel(float* __restrict__ arr, int* list_a,
+                                                      int* list_b, const int* count0, int nx,
+                                                      int ny, int* sweeps_out) {
+    const int lane = threadIdx.x;
+    const int pl = nx * ny;
+    const int dr = lane / 9 - 1, ds = (lane / 3) % 3 - 1, dt = lane % 3 - 1;
+    const int noff = lane < 27 ? dr * pl + ds * ny + dt : 0;
+    int* cur = list_a;
+    int* nxt = list_b;
+    int n = __hip_atomic_load(count0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int sweeps = 0;
+    while (n > 0) {
+        ++sweeps;
+        int m = 0, filled = 0;
+        for (int i = 0; i < n; ++i) {
+            const int id = __hip_atomic_load(cur + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            float v = __builtin_nanf("");
+            if (lane < 27)
+                v = __hip_atomic_load(arr + id + noff, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const bool ok = lane < 27 && !isnan(v);
+            const unsigned long long bal = __ballot(ok);
+            const int cnt = __popcll(bal);
+            float res = __builtin_nanf("");
+            if (cnt > 0) {
+                // rank in a stable ascending sort of the valid values (Median, cpp:1035)
+                int rank = 0;
+                for (int j = 0; j < 27; ++j) {
+                    const float u = __shfl(v, j);
+                    const bool okj = (bal >> j) & 1ull;
+                    rank += (okj && (u < v || (u == v && j < lane))) ? 1 : 0;
+                }
+                if (cnt % 2 == 0) {
+                    const double a = value_at_rank(v, ok, rank, cnt / 2 - 1);
+                    const double b = value_at_rank(v, ok, rank, cnt / 2);
+                    res = (float)((a + b) / 2);
+                } else {
+                    res = value_at_rank(v, ok, rank, cnt / 2);
+                }
+            }
+            if (lane == 0) {
+                __hip_atomic_store(arr + id, res, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (isnan(res))
+                    __hip_atomic_store(nxt + m, id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __threadfence();
+            if (isnan(res))
+                ++m;
+            else
+                ++filled;
+        }
+        if (m == 0 || filled == 0) break;
+        int* t = cur;
+        cur = nxt;
+        nxt = t;
+        n = m;
+    }
+    if (lane == 0 && sweeps_out) *sweeps_out = sweeps;
+}
+
+// RoundMatrix (cpp:1221-1227) then round(4095 * (v / 255)) - 1024 in float
+// (cpp:1732-1738); `round` rounds halves away from zero on both sides.
+__global__ void round_hu_kernel(float* __restrict__ arr, long long n, int to_hu) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        float v = roundf(arr[i]);
+        if (to_hu) v = roundf(4095.0f * (v / 255.0f)) - 1024.0f;
+        arr[i] = v;
+    }
+}
+
+static int post_checks(int nh, int nx, int ny, int nz, const float* d_arr) {
+    if (!d_arr || nh < 1 || nx < 1 || ny < 1 || nz < 1) return PCT_ERR_INVALID;
+    if ((long long)nx * ny * nz >= INT_MAX) return PCT_ERR_TOO_LARGE;
+    return PCT_OK;
+}
+
+static int enqueue_fill(int nh, int nx, int ny, int nz, float* d_arr, hipStream_t s) {
+    int dev = 0;
+    int rc = check_device(&dev);
+    if (rc) return rc;
+    const int lo = (nh - 1) / 2 + 1;
+    const long long iz = nz - 2 * lo, ix = nx - 2 * lo, iy = ny - 2 * lo;
+    const long long interior = (iz > 0 && ix > 0 && iy > 0) ? iz * ix * iy : 0;
+    if (interior == 0) return PCT_OK;
+    const size_t list_bytes = ((size_t)interior * 4 + 255) & ~(size_t)255;
+    void* ws = nullptr;
+    rc = ensure_workspace(dev, 2 * list_bytes + 256, &ws);
+    if (rc) return rc;
+    char* w = static_cast<char*>(ws);
+    int* count = reinterpret_cast<int*>(w);
+    int* la = reinterpret_cast<int*>(w + 256);
+    int* lb = reinterpret_cast<int*>(w + 256 + list_bytes);
+    hipLaunchKernelGGL(nan_list_kernel, dim3(1), dim3(1024), 0, s, d_arr, nx, ny, nz, lo, la,
+                       count);
+    hipLaunchKernelGGL(nan_fill_kernel, dim3(1), dim3(64), 0, s, d_arr, la, lb, count, nx, ny,
+                       (int*)nullptr);
+    return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;
+}
+
+}  // namespace pct
+
+using namespace pct;
+
+extern "C" int pct_regularize_dev(int nh, int nx, int ny, int nz, float* d_arr,
+                                  pct_stream_t stream) {
+    const int rc = post_checks(nh, nx, ny, nz, d_arr);
+    if (rc) return rc;
+    return enqueue_fill(nh, nx, ny, nz, d_arr, reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int pct_postprocess_dev(int nh, int nx, int ny, int nz, float* d_arr,
+                                   pct_stream_t stream) {
+    int rc = post_checks(nh, nx, ny, nz, d_arr);
+    if (rc) return rc;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    rc = enqueue_fill(nh, nx, ny, nz, d_arr, s);
+    if (rc) return rc;
+    const long long n = (long long)nx * ny * nz;
+    const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(round_hu_kernel, dim3(blocks), dim3(256), 0, s, d_arr, n, 1);
+    return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;
+}

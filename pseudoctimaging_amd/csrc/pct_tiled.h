@@ -340,5 +340,8 @@ int launch_direct(const pct_problem* pb, const float* img, const float* atlas,
 // Destroys the cached batch graphs (they hold workspace addresses).
 void release_batch_graphs();
 
+// Frees the per-device copies of the embedding network (pct_learned.hip).
+void release_learned_models();
+
 }  // namespace pct
 

@@ -81,16 +81,17 @@ def bf16_words_to_float(words) -> np.ndarray:
 
 def fuse_pruned_device(img, atlas, atlas_label, out, nh: int, sigma: float, beta: float,
                        params: np.ndarray, topk: int = 32, samples: int = 27,
-                       margin: float = 0.0, shape_id: int = -1, stream=None,
-                       report_fallback: bool = True):
+                       margin: float = 0.0, shape_id: int = -1, max_slabs: int = 4,
+                       stream=None, report_fallback: bool = True):
     """Learned-pruned fusion on device tensors.  Returns the fraction of
     (tile, atlas) items that took the full search (synchronises) when
-    report_fallback, else None (asynchronous)."""
+    report_fallback, else None (asynchronous).  max_slabs bounds the dz
+    planes the kept offsets come from (0: any)."""
     params = _f32(params, "params")
     s = _stream(img, stream)
     pb = problem(img.shape, atlas.shape[0], nh, sigma, beta)
     opts = PrunedOpts(topk=int(topk), samples=int(samples), margin=float(margin),
-                      shape_id=int(shape_id))
+                      shape_id=int(shape_id), max_slabs=int(max_slabs))
     frac = ctypes.c_double(-1.0)
     check(lib().pct_fuse_pruned_dev(byref(pb), fptr(params), byref(opts),
                                     _dev_ptr(img, "img"), _dev_ptr(atlas, "atlas"),
