@@ -219,6 +219,8 @@ typedef struct pct_pruned_opts {
     int max_slabs;   /* kept offsets come from at most this many dz planes of
                         the window: the lattice's and the best-scoring others
                         (bounds the LDS staging per tile); 0 = any */
+    int embed_all;   /* 1: embed every atlas voxel instead of only the probe
+                        windows the selection reads (same result; checking) */
 } pct_pruned_opts;
 
 /* Learned-pruned fusion.  Per tile of target voxels and per atlas, the

@@ -66,7 +66,7 @@ class VolumeSet(Structure):
 class PrunedOpts(Structure):
     """pct_pruned_opts"""
     _fields_ = [("topk", c_int), ("samples", c_int), ("margin", c_float), ("shape_id", c_int),
-                ("max_slabs", c_int)]
+                ("max_slabs", c_int), ("embed_all", c_int)]
 
 
 class TrainOpts(Structure):

@@ -45,6 +45,15 @@ constexpr int kSamples = 27;       // 3x3x3 lattice of tail samples
 constexpr int kMaxTopk = 64;
 constexpr int kMaxOffsets = kMaxTopk + kSamples;
 
+// First coordinate of a tile's probe cluster along one axis: the cluster of
+// c voxels (4 along y, 2 along x and z) sits at the centre of the tile's part
+// of the interior [r, n - r), moved inside when the tile is thin.
+__host__ __device__ __forceinline__ int probe_origin(int t, int ext, int r, int n, int c) {
+    const int p0 = r + t * ext;
+    const int pe = min(p0 + ext, n - r);
+    return max(r, min((p0 + pe) / 2 - c / 2, n - r - c));
+}
+
 // ---------------------------------------------------------------------------
 // embedding
 // ---------------------------------------------------------------------------
@@ -227,6 +236,164 @@ __global__ __launch_bounds__(kEmbWaves * 64) void embed_kernel(EmbedArgs a) {
     }
 }
 
+constexpr int kWinRow = 36;   // LDS row length of the window kernel (>= 4 + 2*7 + 2 + 16)
+
+struct EmbedWinArgs {
+    const float* vol;               // [nvol][nz][nx][ny]
+    unsigned short* emb;            // [nvol][nz][nx][ny][16] BF16 (window voxels written)
+    const v4s* frag;
+    const float* bias;
+    int nx, ny, nz, nvol;
+    int r, hw;
+    int tiles_x, tiles_y, oy, wx;   // tile geometry of the selection
+    int zlo, nzr;                   // z range [zlo, zlo + nzr) covering every window
+    int jobs;                       // nvol * tiles_x * tiles_y * nzr
+    float mu, inv_sd;
+};
+
+// Embeddings of the atlas voxels the selection reads: for every tile column
+// (tx, ty) the probe window's x and y range, over every z any window of the
+// column reaches.  One wave per (volume, tile column, z): it stages the 3 x
+// (window rows + 2) neighbour rows in LDS and runs the MFMA chain of
+// embed_kernel on 16 voxels at a time, with the same feature arithmetic, so
+// the values equal embed_kernel's bit for bit.
+__global__ __launch_bounds__(kEmbWaves * 64) void embed_windows_kernel(EmbedWinArgs a) {
+    __shared__ float rows[kEmbWaves][3 * 18 * kWinRow];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, n = lane & 15;
+    v4s w1[4][2], w2[4][4], w3[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) w1[t][s] = a.frag[(t * 2 + s) * 64 + lane];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) w2[t][s] = a.frag[(8 + t * 4 + s) * 64 + lane];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) w3[s] = a.frag[(24 + s) * 64 + lane];
+    v4f b1[4], b2[4], b3;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            b1[t][j] = a.bias[16 * t + 4 * g + j];
+            b2[t][j] = a.bias[64 + 16 * t + 4 * g + j];
+        }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b3[j] = a.bias[128 + 4 * g + j];
+
+    float* __restrict__ R = rows[wv];
+    const long long vsize = (long long)a.nx * a.ny * a.nz;
+    for (int base = blockIdx.x * kEmbWaves; base < a.jobs; base += gridDim.x * kEmbWaves) {
+        const int job = base + wv;
+        const bool active = job < a.jobs;
+        int z = 0, tx = 0, ty = 0, vi = 0;
+        if (active) {
+            z = a.zlo + job % a.nzr;
+            int t = job / a.nzr;
+            ty = t % a.tiles_y;
+            t /= a.tiles_y;
+            tx = t % a.tiles_x;
+            vi = t / a.tiles_x;
+        }
+        const int px0 = probe_origin(tx, a.wx, a.r, a.nx, 2);
+        const int py0 = probe_origin(ty, a.oy, a.r, a.ny, 4);
+        const int xa = max(px0 - a.hw, 0), xb = min(px0 + 1 + a.hw, a.nx - 1);
+        const int ya = max(py0 - a.hw, 0), yb = min(py0 + 3 + a.hw, a.ny - 1);
+        const int nxr = xb - xa + 3, leny = yb - ya + 1;
+        __syncthreads();   // the previous job's reads of R are done
+        if (active) {
+            const float* __restrict__ v = a.vol + vi * vsize;
+            const int cols = leny + 2, total = 3 * nxr * cols;
+            for (int e = lane; e < total; e += 64) {
+                const int q = e / cols, c = e - (e / cols) * cols;
+                const int zz = z + q / nxr - 1, xx = xa - 1 + q % nxr, yy = ya - 1 + c;
+                float t = 0.f;
+                if (zz >= 0 && zz < a.nz && xx >= 0 && xx < a.nx && yy >= 0 && yy < a.ny)
+                    t = (v[((long long)zz * a.nx + xx) * a.ny + yy] - a.mu) * a.inv_sd;
+                R[q * kWinRow + c] = t;
+            }
+        }
+        __syncthreads();
+        if (!active) continue;
+        // gather plan: f = sc * R[oa + o] - sd * R[ob + o], o = xl * kWinRow + yl
+        int oa[2][4], ob[2][4];
+        float sc[2][4], sd[2][4];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k = 16 * s + 4 * g + j;
+                int o1 = 0, o2 = 0;
+                float c1 = 0.f, c2 = 0.f;
+                if (k < 27) {
+                    o1 = ((k / 9) * nxr + (k / 3) % 3) * kWinRow + k % 3;
+                    c1 = 1.f;
+                } else if (k == 27) {
+                    o1 = (2 * nxr + 1) * kWinRow + 1;
+                    o2 = (0 * nxr + 1) * kWinRow + 1;
+                    c1 = c2 = 0.5f;
+                } else if (k == 28) {
+                    o1 = (1 * nxr + 2) * kWinRow + 1;
+                    o2 = (1 * nxr + 0) * kWinRow + 1;
+                    c1 = c2 = 0.5f;
+                } else if (k == 29) {
+                    o1 = (1 * nxr + 1) * kWinRow + 2;
+                    o2 = (1 * nxr + 1) * kWinRow + 0;
+                    c1 = c2 = 0.5f;
+                }
+                oa[s][j] = o1;
+                ob[s][j] = o2;
+                sc[s][j] = c1;
+                sd[s][j] = c2;
+            }
+        for (int xl = 0; xl <= xb - xa; ++xl) {
+            unsigned short* __restrict__ dst0 =
+                a.emb + (vi * vsize + ((long long)z * a.nx + xa + xl) * a.ny + ya) * 16 + 4 * g;
+            for (int c0 = 0; c0 < leny; c0 += 16) {
+                const int yl = c0 + n;
+                const int o = xl * kWinRow + yl;
+                float f[2][4];
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        f[s][j] = fmaf(-sd[s][j], R[ob[s][j] + o], sc[s][j] * R[oa[s][j] + o]);
+                const v4s xb0 = to_bf16(f[0][0], f[0][1], f[0][2], f[0][3]);
+                const v4s xb1 = to_bf16(f[1][0], f[1][1], f[1][2], f[1][3]);
+                v4s h1b[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    v4f acc = b1[t];
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[t][0], xb0, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[t][1], xb1, acc, 0, 0, 0);
+                    h1b[t] = relu_bf16(acc);
+                }
+                v4s h2b[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    v4f acc = b2[t];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s)
+                        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2[t][s], h1b[s], acc, 0,
+                                                                          0, 0);
+                    h2b[t] = relu_bf16(acc);
+                }
+                v4f e = b3;
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    e = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3[s], h2b[s], e, 0, 0, 0);
+                if (yl < leny) {
+                    const v4s eb = to_bf16(e[0], e[1], e[2], e[3]);
+                    *reinterpret_cast<uint2*>(dst0 + (long long)yl * 16) =
+                        __builtin_bit_cast(uint2, eb);
+                }
+            }
+        }
+    }
+}
+
 // Weight fragments in MFMA A-operand order: fragment (layer, M-tile t,
 // K-step s), lane l, element j holds W[k = 16 s + 4 (l / 16) + j][m = 16 t + l % 16].
 static void make_fragments(const float* p, std::vector<unsigned short>& frag,
@@ -339,6 +506,46 @@ static int launch_embed(const float* params, const void* d_model, const float* v
     return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;
 }
 
+// Embeddings inside the selection's probe windows (see embed_windows_kernel):
+// `nvol` volumes, window half-width `whw` around each probe cluster (the
+// search half window for the atlases, 0 for the target, whose probe
+// clusters alone are read); tile geometry as in pct_fuse_pruned_dev.
+static int launch_embed_windows(const float* params, const void* d_model, const float* vol,
+                                int nvol, int whw, const pct_problem* pb, const TileGeometry& g,
+                                int oy, int wx, int bz, unsigned short* emb, hipStream_t stream) {
+    EmbedWinArgs a;
+    a.vol = vol;
+    a.emb = emb;
+    a.frag = static_cast<const v4s*>(d_model);
+    a.bias = reinterpret_cast<const float*>(static_cast<const char*>(d_model) + kFrags * 64 * 8);
+    a.nx = pb->nx;
+    a.ny = pb->ny;
+    a.nz = pb->nz;
+    a.nvol = nvol;
+    a.r = g.r;
+    a.hw = whw;
+    a.tiles_x = g.tiles_x;
+    a.tiles_y = g.tiles_y;
+    a.oy = oy;
+    a.wx = wx;
+    if (g.n_tiles == 0) return PCT_OK;
+    const int z_first = probe_origin(0, bz, g.r, pb->nz, 2);
+    const int z_last = probe_origin(g.tiles_z - 1, bz, g.r, pb->nz, 2);
+    a.zlo = std::max(z_first - whw, 0);
+    const int zhi = std::min(z_last + 1 + whw, pb->nz - 1);
+    a.nzr = zhi - a.zlo + 1;
+    const long long jobs = (long long)a.nvol * a.tiles_x * a.tiles_y * a.nzr;
+    if (jobs >= INT_MAX) return PCT_ERR_TOO_LARGE;
+    a.jobs = (int)jobs;
+    a.mu = params[kOffMu];
+    a.inv_sd = params[kOffSd];
+    const long long blocks = std::min<long long>((jobs + kEmbWaves - 1) / kEmbWaves, 256LL * 16);
+    if (blocks > 0)
+        hipLaunchKernelGGL(embed_windows_kernel, dim3((unsigned)blocks), dim3(kEmbWaves * 64), 0,
+                           stream, a);
+    return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;
+}
+
 // ---------------------------------------------------------------------------
 // window box sums of the atlas CT
 // ---------------------------------------------------------------------------
@@ -427,6 +634,14 @@ __global__ void box_line(const float* __restrict__ in, float* __restrict__ out, 
     }
 }
 
+// out = in * scale (the atlas MRI as the tiled kernels stage it).
+__global__ void scale_copy(const float* __restrict__ in, float* __restrict__ out, long long n,
+                           float scale) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        out[i] = in[i] * scale;
+}
+
 // ---------------------------------------------------------------------------
 // candidate selection
 // ---------------------------------------------------------------------------
@@ -447,9 +662,10 @@ struct SelectArgs {
     int lattice_dz[3];            // dz of the lattice samples (always allowed)
 };
 
-constexpr int kSelThreads = 256;
-constexpr int kSelPerThread = 16;   // 4096 offsets (NH = 15: 3375)
+constexpr int kSelThreads = 512;
+constexpr int kSelPerThread = 8;    // 4096 offsets (NH = 15: 3375)
 constexpr int kNoOffset = 1 << 30;
+constexpr int kSelBatch = 8;        // window loads in flight per thread
 
 // One workgroup per item = (atlas, tile), item = atlas * n_tiles + tile.
 __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
@@ -468,62 +684,85 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
     const int ncand = nw2 * nw;
     // probe cluster: 4 (y) x 2 (x) x 2 (z) voxels at the centre of the
     // tile's part of the interior
-    const int y0 = r + ty * a.oy, x0 = r + tx * a.wx, z0 = r + tz * a.bz;
-    const int ye = min(y0 + a.oy, a.ny - r), xe = min(x0 + a.wx, a.nx - r),
-              ze = min(z0 + a.bz, a.nz - r);
-    const int py0 = max(r, min((y0 + ye) / 2 - 2, a.ny - r - 4));
-    const int px0 = max(r, min((x0 + xe) / 2 - 1, a.nx - r - 2));
-    const int pz0 = max(r, min((z0 + ze) / 2 - 1, a.nz - r - 2));
+    const int py0 = probe_origin(ty, a.oy, r, a.ny, 4);
+    const int px0 = probe_origin(tx, a.wx, r, a.nx, 2);
+    const int pz0 = probe_origin(tz, a.bz, r, a.nz, 2);
     // LDS: probe target embeddings (16 x 16 floats), then the atlas
     // embeddings of the cluster's window as 8 planes of BF16 pairs (plane k2
     // holds dimensions 2 k2, 2 k2 + 1 of every window voxel: lanes scoring
     // neighbouring offsets read neighbouring words).  Coordinates are clamped
     // into the volume (a thin interior can leave the window partly outside).
-    float* sT = sh;
+    unsigned* sU = reinterpret_cast<unsigned*>(sh);          // [16 probes][8] BF16 pairs of -2 t
     unsigned* sA = reinterpret_cast<unsigned*>(sh + 256);
     const int WY = 4 + 2 * hw, WXX = 2 + 2 * hw, WZ = 2 + 2 * hw;
     const int nwin = WY * WXX * WZ;
     const long long plane = (long long)a.nx * a.ny;
-    for (int i = threadIdx.x; i < 256; i += kSelThreads) {
-        const int p = i >> 4, k = i & 15;
+    for (int i = threadIdx.x; i < 128; i += kSelThreads) {
+        const int p = i >> 3, k2 = i & 7;
         const int yy = min(py0 + (p & 3), a.ny - 1), xx = min(px0 + ((p >> 2) & 1), a.nx - 1),
                   zz = min(pz0 + (p >> 3), a.nz - 1);
-        sT[i] = bf2f(a.embT[(zz * plane + (long long)xx * a.ny + yy) * 16 + k]);
+        const unsigned short* e = a.embT + (zz * plane + (long long)xx * a.ny + yy) * 16 + 2 * k2;
+        // -2 t is exact in BF16
+        const unsigned lo = f2bf(-2.f * bf2f(e[0])), hi = f2bf(-2.f * bf2f(e[1]));
+        sU[i] = lo | (hi << 16);
     }
-    const unsigned* __restrict__ eA =
-        reinterpret_cast<const unsigned*>(a.embA + (long long)at * a.N * 16);
-    for (int i = threadIdx.x; i < nwin * 8; i += kSelThreads) {
-        const int w = i >> 3, k2 = i & 7;
-        const int wy = w % WY, wx = (w / WY) % WXX, wz = w / (WY * WXX);
-        const int yy = min(max(py0 - hw + wy, 0), a.ny - 1);
-        const int xx = min(max(px0 - hw + wx, 0), a.nx - 1);
-        const int zz = min(max(pz0 - hw + wz, 0), a.nz - 1);
-        sA[k2 * nwin + w] = eA[(zz * plane + (long long)xx * a.ny + yy) * 8 + k2];
+    // two 16-byte loads per window voxel, kSelBatch of them in flight per
+    // thread before any LDS store
+    const uint4* __restrict__ eA =
+        reinterpret_cast<const uint4*>(a.embA + (long long)at * a.N * 16);
+    for (int i0 = threadIdx.x; i0 < nwin * 2; i0 += kSelThreads * kSelBatch) {
+        uint4 v[kSelBatch];
+#pragma unroll
+        for (int b = 0; b < kSelBatch; ++b) {
+            const int i = min(i0 + b * kSelThreads, nwin * 2 - 1);
+            const int w = i >> 1, h = i & 1;
+            const int wy = w % WY, wx = (w / WY) % WXX, wz = w / (WY * WXX);
+            const int yy = min(max(py0 - hw + wy, 0), a.ny - 1);
+            const int xx = min(max(px0 - hw + wx, 0), a.nx - 1);
+            const int zz = min(max(pz0 - hw + wz, 0), a.nz - 1);
+            v[b] = eA[(zz * plane + (long long)xx * a.ny + yy) * 2 + h];
+        }
+#pragma unroll
+        for (int b = 0; b < kSelBatch; ++b) {
+            const int i = i0 + b * kSelThreads;
+            if (i < nwin * 2) {
+                const int w = i >> 1, k0 = (i & 1) * 4;
+                sA[(k0 + 0) * nwin + w] = v[b].x;
+                sA[(k0 + 1) * nwin + w] = v[b].y;
+                sA[(k0 + 2) * nwin + w] = v[b].z;
+                sA[(k0 + 3) * nwin + w] = v[b].w;
+            }
+        }
     }
     __syncthreads();
-    // scores: thread handles offsets c = threadIdx.x + q * kSelThreads
+    // scores: thread handles offsets c = threadIdx.x + q * kSelThreads; the
+    // probe loop stays rolled so the unrolled q loop keeps a small code body
     float sc[kSelPerThread];
 #pragma unroll
     for (int q = 0; q < kSelPerThread; ++q) {
         const int c = threadIdx.x + q * kSelThreads;
         float score = __builtin_huge_valf();
         if (c < ncand) {
+            // sum over the probes of |a|^2 - 2 t.a, which orders the offsets
+            // like the sum of |t - a|^2 (the |t|^2 terms are common to every
+            // offset); two BF16 dot products per dimension pair
             const int dz = c / nw2 - hw, dx = (c / nw) % nw - hw, dy = c % nw - hw;
-            score = 0.f;
+            float acc[4] = {0.f, 0.f, 0.f, 0.f};   // independent chains
+#pragma unroll 1
             for (int p = 0; p < 16; ++p) {
                 const int wy = (p & 3) + hw + dy, wx = ((p >> 2) & 1) + hw + dx,
                           wz = (p >> 3) + hw + dz;
                 const unsigned* ea = sA + (wz * WXX + wx) * WY + wy;
-                const float* et = sT + p * 16;
+                const unsigned* eu = sU + p * 8;
 #pragma unroll
                 for (int k2 = 0; k2 < 8; ++k2) {
-                    const unsigned u = ea[k2 * nwin];
-                    const float d0 = et[2 * k2] - __uint_as_float(u << 16);
-                    const float d1 = et[2 * k2 + 1] - __uint_as_float(u & 0xffff0000u);
-                    score = fmaf(d0, d0, score);
-                    score = fmaf(d1, d1, score);
+                    const bf2 av = __builtin_bit_cast(bf2, ea[k2 * nwin]);
+                    const bf2 uv = __builtin_bit_cast(bf2, eu[k2]);
+                    acc[k2 & 3] = __builtin_amdgcn_fdot2_f32_bf16(av, av, acc[k2 & 3], false);
+                    acc[k2 & 3] = __builtin_amdgcn_fdot2_f32_bf16(av, uv, acc[k2 & 3], false);
                 }
             }
+            score = (acc[0] + acc[1]) + (acc[2] + acc[3]);
         }
         sc[q] = score;
     }
@@ -535,12 +774,17 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
         if (c < ncand) key[c] = sc[q];
     }
     __syncthreads();
-    // best score of every dz slab, then the allowed slabs: the lattice's and
-    // the best others up to max_slabs (ties to the smaller dz)
-    if (threadIdx.x < nw) {
+    // best score of every dz slab (16 threads per slab, then a shuffle
+    // reduction), then the allowed slabs: the lattice's and the best others
+    // up to max_slabs (ties to the smaller dz)
+    {
+        const int d = threadIdx.x >> 4, part = threadIdx.x & 15;
         float m = __builtin_huge_valf();
-        for (int c = threadIdx.x * nw2; c < (threadIdx.x + 1) * nw2; ++c) m = fminf(m, key[c]);
-        dzmin[threadIdx.x] = m;
+        if (d < nw)
+            for (int c = d * nw2 + part; c < (d + 1) * nw2; c += 16) m = fminf(m, key[c]);
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off, 16));
+        if (d < nw && part == 0) dzmin[d] = m;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -637,6 +881,7 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
 
 struct PrunedArgs {
     TiledArgs t;                    // problem, tiles (group = 1), workspace partials
+    const float* atlas_scaled;      // [n][N] atlas MRI times t.scale
     const int* sel;                 // [items][topk]
     const unsigned char* flag;      // [items]
     const float* boxL;              // [n][N] window sums of the CT
@@ -712,6 +957,7 @@ __global__ __launch_bounds__(NW * 64) void pruned_kernel(PrunedArgs pa) {
     extern __shared__ float lds[];
     __shared__ int offs[kMaxOffsets];     // merged offset list, ascending
     __shared__ int is_sample[kMaxOffsets];
+    __shared__ int kept[kMaxTopk], smp[kSamples];
     __shared__ int n_offs, n_samp_eff;
     const TiledArgs& args = pa.t;
     const int item = xcd_contiguous(blockIdx.x, args.n_items);
@@ -723,14 +969,17 @@ __global__ __launch_bounds__(NW * 64) void pruned_kernel(PrunedArgs pa) {
     const int tx = t % args.tiles_x;
     const int tz = t / args.tiles_x;
 
+    // both lists into LDS with one load per thread, then one thread merges
+    if (threadIdx.x < pa.topk) kept[threadIdx.x] = pa.sel[(long long)item * pa.topk + threadIdx.x];
+    if (threadIdx.x < pa.n_samples) smp[threadIdx.x] = pa.samples[threadIdx.x];
+    __syncthreads();
     if (threadIdx.x == 0) {
         // merge the kept offsets with the lattice samples (both ascending);
         // a sample that is also kept counts once, as kept
-        const int* ks = pa.sel + (long long)item * pa.topk;
         int i = 0, j = 0, m = 0, ns = 0;
         for (;;) {
-            const int a = i < pa.topk ? ks[i] : kNoOffset;
-            const int b = j < pa.n_samples ? pa.samples[j] : kNoOffset;
+            const int a = i < pa.topk ? kept[i] : kNoOffset;
+            const int b = j < pa.n_samples ? smp[j] : kNoOffset;
             if (a >= kNoOffset && b >= kNoOffset) break;
             if (a < b) {
                 offs[m] = a; is_sample[m] = 0; ++m; ++i;
@@ -760,14 +1009,7 @@ __global__ __launch_bounds__(NW * 64) void pruned_kernel(PrunedArgs pa) {
     const int xb = x0 + sg * BX;
 
     float tv[BZ + 2][BX + 2];
-#pragma unroll
-    for (int pz = 0; pz < BZ + 2; ++pz)
-#pragma unroll
-        for (int i = 0; i < BX + 2; ++i) {
-            const int zz = z0 - 1 + pz, xx = xb - 1 + i;
-            const bool ok = zz < nz && xx < nx && y < ny;
-            tv[pz][i] = ok ? args.img[zz * plane + (long long)xx * ny + y] * args.scale : 0.f;
-        }
+    load_target_block<BX, BZ>(args.img, args.scale, z0, xb, y, nx, ny, nz, plane, tv);
     float P[BZ][BX], S[BZ][BX], Le[BZ][BX], Ws[BZ][BX];
 #pragma unroll
     for (int j = 0; j < BZ; ++j)
@@ -778,15 +1020,16 @@ __global__ __launch_bounds__(NW * 64) void pruned_kernel(PrunedArgs pa) {
     const int xs = x0 - 1 - HW;
     const int ys = y0 - 1 - HW;
     constexpr int NWIN = 2 * HW + 1;
-    const float* __restrict__ A = args.atlas + (long long)at * args.N;
+    DirectPlan<C, NW> dp;
+    dp.init(xs, ys, nx, ny, wv, lane);
+    const float* __restrict__ As = pa.atlas_scaled + (long long)at * args.N;
     const float* __restrict__ L = args.label + (long long)at * args.N;
     const int m = n_offs;
     int k = 0;
     while (k < m) {
         const int dz = offs[k] / (NWIN * NWIN) - HW;
         __syncthreads();
-        stage_slab<C, NW>(lds, A, L, args.scale, z0 + dz, xs, ys, nx, ny, nz, plane, wv, lane);
-        __syncthreads();
+        stage_slab_direct<C, NW>(lds, As, L, z0 + dz, nz, plane, wv, dp);
         for (; k < m && offs[k] / (NWIN * NWIN) - HW == dz; ++k) {
             const int dx = (offs[k] / NWIN) % NWIN, dy = offs[k] % NWIN;   // 0 .. 2HW
             const int base = lane_base + dx * C::CW + dy;
@@ -923,6 +1166,7 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     const size_t o_tmp = off; off = align(off + sizeof(float) * (size_t)N * n);
     const size_t o_sel = off; off = align(off + sizeof(int) * (size_t)items * topk);
     const size_t o_flag = off; off = align(off + (size_t)items);
+    const size_t o_scaled = off; off = align(off + sizeof(float) * (size_t)N * n);
     void* ws = nullptr;
     rc = ensure_workspace(dev, off, &ws);
     if (rc) return rc;
@@ -934,8 +1178,16 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     // 1. embeddings of the target and of every atlas MRI
     unsigned short* embT = reinterpret_cast<unsigned short*>(w + o_embT);
     unsigned short* embA = reinterpret_cast<unsigned short*>(w + o_embA);
-    rc = launch_embed(params, model, d_img, pb->nx, pb->ny, pb->nz, 1, embT, s);
-    if (!rc) rc = launch_embed(params, model, d_atlas, pb->nx, pb->ny, pb->nz, n, embA, s);
+    const int t_oy = sh.seg_width - 2, t_wx = sh.waves * (64 / sh.seg_width) * sh.bx;
+    if (opts->embed_all) {
+        rc = launch_embed(params, model, d_img, pb->nx, pb->ny, pb->nz, 1, embT, s);
+        if (!rc) rc = launch_embed(params, model, d_atlas, pb->nx, pb->ny, pb->nz, n, embA, s);
+    } else {
+        rc = launch_embed_windows(params, model, d_img, 1, 0, pb, g, t_oy, t_wx, sh.bz, embT, s);
+        if (!rc)
+            rc = launch_embed_windows(params, model, d_atlas, n, hw, pb, g, t_oy, t_wx, sh.bz,
+                                      embA, s);
+    }
     if (rc) return rc;
 
     // 2. window sums of every atlas CT: y into tmp, x into boxL, z into tmp
@@ -1004,9 +1256,14 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
         if (!seen) sa.lattice_dz[sa.n_lattice_dz++] = dz;
     }
     if (ncand > kSelThreads * kSelPerThread) return PCT_ERR_UNSUPPORTED;
+    // largest compacted candidate count: every slab, or the allowed ones
+    const int slabs = (opts->max_slabs <= 0 || opts->max_slabs >= nw)
+                          ? nw
+                          : std::min(nw, std::max(opts->max_slabs, sa.n_lattice_dz));
     int p2 = 1;
-    while (p2 < ncand) p2 <<= 1;
+    while (p2 < slabs * nw * nw) p2 <<= 1;
     const int win = (4 + 2 * hw) * (2 + 2 * hw) * (2 + 2 * hw);
+    // phase 1: probe embeddings and window; phase 2: scores and sort arrays
     const size_t sel_lds = std::max(256 * 4 + (size_t)win * 32,
                                     (size_t)((ncand + 63) & ~63) * 4 + (size_t)p2 * 8);
     if (sel_lds > (size_t)kLdsPerCu) return PCT_ERR_UNSUPPORTED;
@@ -1026,6 +1283,14 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     g1.n_groups = n;
     g1.n_items = items;
     pa.t = make_tiled_args(pb, g1, d_img, d_atlas, d_label, part);
+    float* scaled = reinterpret_cast<float*>(w + o_scaled);
+    {
+        const unsigned bs = (unsigned)std::min<long long>((N * n + 255) / 256, 65536);
+        hipLaunchKernelGGL(scale_copy, dim3(bs), dim3(256), 0, s, d_atlas, scaled, N * n,
+                           pa.t.scale);
+        if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
+    }
+    pa.atlas_scaled = scaled;
     pa.sel = sa.sel;
     pa.flag = sa.flag;
     pa.boxL = boxL;

@@ -203,6 +203,92 @@ __device__ __forceinline__ void stage_slab(float* __restrict__ lds, const float*
     }
 }
 
+// Zeros read by lanes whose slab position lies outside the volume.
+__device__ const float kZeros[64] = {};
+
+// The slab of stage_slab, copied with direct global-to-LDS loads
+// (global_load_lds_dword): every lane of a wave instruction fills one of 64
+// consecutive LDS floats of a plane, so a plane is PS/64 instructions, all of
+// the slab's loads are in flight at once, and no VGPR holds the data.  Wave w
+// copies chunks j = w, w + NW, ... of every plane, so a lane's in-plane
+// source offsets depend only on the tile: DirectPlan computes them once per
+// item, and a copy costs one address add per instruction.
+template <class C, int NW>
+struct DirectPlan {
+    static constexpr int CH = (C::PS + 63) / 64;      // 64-float chunks per plane
+    static constexpr int JN = (CH + NW - 1) / NW;     // chunks per wave and plane
+    // x * ny + y of the lane's source voxel within a plane, -1 outside the
+    // volume (reads zeros), -2 past the end of the plane (no copy)
+    int off[JN];
+
+    __device__ __forceinline__ void init(int xs, int ys, int nx, int ny, int wv, int lane) {
+#pragma unroll
+        for (int k = 0; k < JN; ++k) {
+            const int j = wv + k * NW;
+            const int p = j * 64 + lane;
+            const int row = p / C::CW, col = p - (p / C::CW) * C::CW;
+            const int xx = xs + row, yy = ys + col;
+            int o = -2;
+            if (j < CH && p < C::PS)
+                o = (xx >= 0 && xx < nx && yy >= 0 && yy < ny) ? xx * ny + yy : -1;
+            off[k] = o;
+        }
+    }
+};
+
+// Copies the slab of stage_slab (the MRI already scaled, `As`); positions
+// outside the volume read zeros.  Ends with every wave's loads complete and a
+// workgroup barrier.
+template <class C, int NW>
+__device__ __forceinline__ void stage_slab_direct(float* __restrict__ lds,
+                                                  const float* __restrict__ As,
+                                                  const float* __restrict__ L, int zdz, int nz,
+                                                  long long plane, int wv,
+                                                  const DirectPlan<C, NW>& dp) {
+    typedef __attribute__((address_space(3))) void* lds_ptr;
+    constexpr int NP = C::APLANES + C::LPLANES;
+    using P = DirectPlan<C, NW>;
+#pragma unroll 2
+    for (int pl = 0; pl < NP; ++pl) {
+        const bool is_mri = pl < C::APLANES;
+        const int zz = is_mri ? zdz - 1 + pl : zdz + (pl - C::APLANES);
+        const bool zok = zz >= 0 && zz < nz;
+        const float* base = (is_mri ? As : L) + (zok ? zz : 0) * plane;
+#pragma unroll
+        for (int k = 0; k < P::JN; ++k) {
+            const int j = wv + k * NW;
+            const int o = dp.off[k];
+            if (o >= -1) {
+                const float* src = (zok && o >= 0) ? base + o : kZeros;
+                __builtin_amdgcn_global_load_lds(src, (lds_ptr)(lds + pl * C::PS + j * 64), 4,
+                                                 0, 0);
+            }
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+}
+
+// Scaled target values of a lane's BZ x BX block plus a one-voxel halo in z
+// and x, zero outside the volume.  The loads use clamped (always valid)
+// addresses and are selected afterwards, so they are all issued before the
+// first wait.
+template <int BX, int BZ>
+__device__ __forceinline__ void load_target_block(const float* __restrict__ img, float scale,
+                                                  int z0, int xb, int y, int nx, int ny, int nz,
+                                                  long long plane,
+                                                  float (&tv)[BZ + 2][BX + 2]) {
+    const int yc = min(y, ny - 1);
+#pragma unroll
+    for (int pz = 0; pz < BZ + 2; ++pz)
+#pragma unroll
+        for (int i = 0; i < BX + 2; ++i) {
+            const int zz = z0 - 1 + pz, xx = xb - 1 + i;
+            const float v = img[min(zz, nz - 1) * plane + (long long)min(xx, nx - 1) * ny + yc];
+            tv[pz][i] = (zz < nz && xx < nx && y < ny) ? v * scale : 0.f;
+        }
+}
+
 // One work item (tile, atlas group) of one problem: the whole workgroup runs
 // it.  Shared by the single-problem kernel and the persistent batched kernel,
 // so a problem gives bit-identical results through either.
@@ -236,14 +322,7 @@ __device__ __forceinline__ void tiled_item(const TiledArgs& args, int item,
     // and x (y halo comes from the neighbouring lanes).  Zero outside the
     // volume: such values only reach outputs that are never stored.
     float tv[BZ + 2][BX + 2];
-#pragma unroll
-    for (int pz = 0; pz < BZ + 2; ++pz)
-#pragma unroll
-        for (int i = 0; i < BX + 2; ++i) {
-            const int zz = z0 - 1 + pz, xx = xb - 1 + i;
-            const bool ok = zz < nz && xx < nx && y < ny;
-            tv[pz][i] = ok ? args.img[zz * plane + (long long)xx * ny + y] * args.scale : 0.f;
-        }
+    load_target_block<BX, BZ>(args.img, args.scale, z0, xb, y, nx, ny, nz, plane, tv);
 
     float P[BZ][BX], S[BZ][BX];
 #pragma unroll

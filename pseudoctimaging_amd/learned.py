@@ -82,7 +82,7 @@ def bf16_words_to_float(words) -> np.ndarray:
 def fuse_pruned_device(img, atlas, atlas_label, out, nh: int, sigma: float, beta: float,
                        params: np.ndarray, topk: int = 32, samples: int = 27,
                        margin: float = 0.0, shape_id: int = -1, max_slabs: int = 4,
-                       stream=None, report_fallback: bool = True):
+                       embed_all: bool = False, stream=None, report_fallback: bool = True):
     """Learned-pruned fusion on device tensors.  Returns the fraction of
     (tile, atlas) items that took the full search (synchronises) when
     report_fallback, else None (asynchronous).  max_slabs bounds the dz
@@ -91,7 +91,8 @@ def fuse_pruned_device(img, atlas, atlas_label, out, nh: int, sigma: float, beta
     s = _stream(img, stream)
     pb = problem(img.shape, atlas.shape[0], nh, sigma, beta)
     opts = PrunedOpts(topk=int(topk), samples=int(samples), margin=float(margin),
-                      shape_id=int(shape_id), max_slabs=int(max_slabs))
+                      shape_id=int(shape_id), max_slabs=int(max_slabs),
+                      embed_all=int(bool(embed_all)))
     frac = ctypes.c_double(-1.0)
     check(lib().pct_fuse_pruned_dev(byref(pb), fptr(params), byref(opts),
                                     _dev_ptr(img, "img"), _dev_ptr(atlas, "atlas"),

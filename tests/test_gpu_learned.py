@@ -165,6 +165,24 @@ def test_pruned_full_fallback_is_bit_exact(gpu):
     assert np.array_equal(out.cpu().numpy().view(np.uint32), ref.cpu().numpy().view(np.uint32))
 
 
+@pytest.mark.parametrize("shape", [(30, 40, 44), (16, 17, 50), (40, 70, 21)])
+def test_window_embedding_equals_full(gpu, shape):
+    # The pruned path embeds only the atlas voxels the selection reads; with
+    # every voxel embedded instead, the result is bit-identical.
+    import torch
+    img, amri, lab = _fusion_case(shape, 2, 11, 23)
+    p = _random_params(9, *learned.intensity_stats(img))
+    dimg, dam, dlab = _dev(img), _dev(amri), _dev(lab)
+    outs = []
+    for full in (False, True):
+        out = torch.full(img.shape, -9.0, device="cuda:0")
+        learned.fuse_pruned_device(dimg, dam, dlab, out, 11, 2.5, 1.0, p, topk=16, samples=27,
+                                   margin=0.05, embed_all=full)
+        torch.cuda.synchronize()
+        outs.append(out.cpu().numpy())
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
 def _hu_mae(a, b):
     ok = np.isfinite(a) & np.isfinite(b)
     return float(np.abs(4095.0 * (a[ok] - b[ok]) / 255.0).mean()), ok
