@@ -56,12 +56,24 @@ def cpu_baseline(img, amri, lab, nh, sigma, beta, planes, rows, threads):
     camri = np.ascontiguousarray(amri[:, :zc, :xc, :])
     clab = np.ascontiguousarray(lab[:, :zc, :xc, :])
     kind = "reference" if oracle.ref_available() else "port"
-    t0 = time.perf_counter()
-    if kind == "reference":
-        oracle.ref_fuse(cimg, camri, clab, nh, sigma, beta, threads=threads)
-    else:
-        oracle.fuse(cimg, camri, clab, nh, sigma, beta, threads=threads)
-    dt = time.perf_counter() - t0
+    # the reference program prints progress to stdout; keep stdout for the
+    # one JSON line by pointing file descriptor 1 at stderr meanwhile
+    import ctypes
+    libc = ctypes.CDLL(None)
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        t0 = time.perf_counter()
+        if kind == "reference":
+            oracle.ref_fuse(cimg, camri, clab, nh, sigma, beta, threads=threads)
+        else:
+            oracle.fuse(cimg, camri, clab, nh, sigma, beta, threads=threads)
+        dt = time.perf_counter() - t0
+    finally:
+        libc.fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
     vox_sample = planes * rows * interior(img.shape[2], r)
     return kind, dt, vox_sample
 
