@@ -78,8 +78,11 @@ struct TiledCfg {
     static constexpr int COLCHUNKS = (CW + 63) / 64;
     // ds_read_b32 takes a 16-bit byte offset: planes are addressed from
     // several base registers, PPB planes (plus row offsets) per base.
-    static constexpr int PPB = (16383 - (BX + 1) * CW) / PS + 1;
+    // The immediate also carries the candidate's dy (up to 2 HW).
+    static constexpr int PPB = (16383 - (BX + 1) * CW - 2 * HW) / PS + 1;
     static_assert(PPB >= 1, "one plane must fit an LDS immediate window");
+    static constexpr int NBA = (APLANES + PPB - 1) / PPB;   // base registers, MRI planes
+    static constexpr int NBL = (LPLANES + PPB - 1) / PPB;   // base registers, CT planes
     static_assert(64 % W == 0, "segment width must divide the wavefront");
     static_assert(BX % 2 == 0 && BZ % 2 == 0, "pairwise box sums need even blocks");
 };
@@ -92,26 +95,32 @@ __device__ __forceinline__ int opaque(int v) {
     return v;
 }
 
-// One (dx, dy) candidate step for the lane's BZ x BX block.  `base` is the
-// LDS element index of plane 0, row 0 of the lane's window for this step.
-template <class C, int BX, int BZ>
-__device__ __forceinline__ void tiled_step(const float* __restrict__ lds, int base,
-                                           const float (&tv)[BZ + 2][BX + 2],
-                                           float (&P)[BZ][BX], float (&S)[BZ][BX]) {
-    constexpr int NBA = (BZ + 2 + C::PPB - 1) / C::PPB;   // bases for the MRI planes
-    constexpr int NBL = (BZ + C::PPB - 1) / C::PPB;       // bases for the CT planes
-    int ba[NBA];
-    int bl[NBL];
+// LDS base registers of one candidate row: bases for the MRI planes (PPB
+// planes each) and for the CT planes, hidden from the optimiser so that each
+// stays one register and the plane, row and dy offsets ride in the ds_read
+// immediates.  `base` is the LDS element index of plane 0, row 0 of the
+// lane's window for dy = 0.
+template <class C>
+__device__ __forceinline__ void make_bases(int base, int (&ba)[C::NBA], int (&bl)[C::NBL]) {
 #pragma unroll
-    for (int b = 0; b < NBA; ++b) ba[b] = b ? opaque(base + b * C::PPB * C::PS) : base;
+    for (int b = 0; b < C::NBA; ++b) ba[b] = opaque(base + b * C::PPB * C::PS);
 #pragma unroll
-    for (int b = 0; b < NBL; ++b) bl[b] = opaque(base + C::LOFF + b * C::PPB * C::PS);
+    for (int b = 0; b < C::NBL; ++b) bl[b] = opaque(base + C::LOFF + b * C::PPB * C::PS);
+}
 
+// One (dx, dy) candidate step for the lane's BZ x BX block, dy = `dy`
+// elements past the bases (a compile-time constant where the dy loop is
+// unrolled).
+template <class C, int BX, int BZ>
+__device__ __forceinline__ void tiled_step_at(const float* __restrict__ lds,
+                                              const int (&ba)[C::NBA], const int (&bl)[C::NBL],
+                                              int dy, const float (&tv)[BZ + 2][BX + 2],
+                                              float (&P)[BZ][BX], float (&S)[BZ][BX]) {
     float ex[BZ + 2][BX];
 #pragma unroll
     for (int pz = 0; pz < BZ + 2; ++pz) {
         float d[BX + 2];
-        const int pp = ba[pz / C::PPB] + (pz % C::PPB) * C::PS;
+        const int pp = ba[pz / C::PPB] + (pz % C::PPB) * C::PS + dy;
 #pragma unroll
         for (int i = 0; i < BX + 2; ++i) {
             const float df = tv[pz][i] - lds[pp + i * C::CW];
@@ -135,15 +144,26 @@ __device__ __forceinline__ void tiled_step(const float* __restrict__ lds, int ba
             const float s1 = lane_next(e1) + (lane_prev(e1) + e1);
             const float w0 = __builtin_amdgcn_exp2f(-s0);
             const float w1 = __builtin_amdgcn_exp2f(-s1);
-            const float l0 = lds[bl[j / C::PPB] + (j % C::PPB) * C::PS + (i + 1) * C::CW];
+            const float l0 = lds[bl[j / C::PPB] + (j % C::PPB) * C::PS + (i + 1) * C::CW + dy];
             const float l1 =
-                lds[bl[(j + 1) / C::PPB] + ((j + 1) % C::PPB) * C::PS + (i + 1) * C::CW];
+                lds[bl[(j + 1) / C::PPB] + ((j + 1) % C::PPB) * C::PS + (i + 1) * C::CW + dy];
             P[j][i] = fmaf(w0, l0, P[j][i]);
             S[j][i] += w0;
             P[j + 1][i] = fmaf(w1, l1, P[j + 1][i]);
             S[j + 1][i] += w1;
         }
     }
+}
+
+// One (dx, dy) candidate step at LDS element index `base` (plane 0, row 0 of
+// the lane's window for this step).
+template <class C, int BX, int BZ>
+__device__ __forceinline__ void tiled_step(const float* __restrict__ lds, int base,
+                                           const float (&tv)[BZ + 2][BX + 2],
+                                           float (&P)[BZ][BX], float (&S)[BZ][BX]) {
+    int ba[C::NBA], bl[C::NBL];
+    make_bases<C>(base, ba, bl);
+    tiled_step_at<C, BX, BZ>(lds, ba, bl, 0, tv, P, S);
 }
 
 // Copies the slab of one (atlas, dz) stage into LDS: plane p < APLANES holds
@@ -343,12 +363,14 @@ __device__ __forceinline__ void tiled_item(const TiledArgs& args, int item,
             __syncthreads();
             stage_slab<C, NW>(lds, A, L, args.scale, z0 + dz, xs, ys, nx, ny, nz, plane, wv, lane);
             __syncthreads();
+            // (dx, dy) loops stay rolled: unrolling dy removes the per-step
+            // base adds but multiplies the code size, and the batched graph
+            // runs several of these kernels side by side
 #pragma unroll 1
             for (int dx = 0; dx <= 2 * HW; ++dx) {
 #pragma unroll 1
-                for (int dy = 0; dy <= 2 * HW; ++dy) {
+                for (int dy = 0; dy <= 2 * HW; ++dy)
                     tiled_step<C, BX, BZ>(lds, lane_base + dx * C::CW + dy, tv, P, S);
-                }
             }
         }
     }
