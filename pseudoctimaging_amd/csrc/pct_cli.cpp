@@ -16,6 +16,14 @@
 //   device       HIP device ordinal (cu:1973-1975), default 0
 //   --seed S     seed of the random-data mode (the reference seeds rand()
 //                from the clock, cpp:918); default 0
+//
+// GpuTestTime options (the reference fixes them in code, cu:2033-2043):
+//   --sweep-nh A,B,...  window edges to sweep (default: NH)
+//   --many K            K samples of K calls per point (default 3; the
+//                       reference uses its `many`)
+//   --results FILE      append the averaged table in the reference's final
+//                       results format (cu:2105-2116)
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -45,12 +53,37 @@ static double now_ms() {
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+// The reference's effective-bandwidth byte count for one call (cu:2193-2198).
+static double naive_gbs(int nx, int ny, int nz, int num, int nh, double ms) {
+    const double hw = (nh - 1) / 2;
+    const double vox = (nx - hw) * (ny - hw) * (nz - hw);
+    const double rB = 4.0 * vox * ((double)num * nh * nh * nh + 2.0 * num * 27 * nh * nh * nh);
+    const double wB = vox * 4.0;
+    return (rB + wB) / (ms * 1e-3) / 1e9;
+}
+
 int main(int argc, char** argv) {
     std::vector<char*> args;
     unsigned long long seed = 0;
+    std::vector<int> sweep_nh;
+    int many = 3;
+    std::string results_path;
     for (int i = 0; i < argc; ++i) {
         if (std::strcmp(argv[i], "--seed") == 0 && i + 1 < argc) {
             seed = std::strtoull(argv[++i], nullptr, 10);
+            continue;
+        }
+        if (std::strcmp(argv[i], "--sweep-nh") == 0 && i + 1 < argc) {
+            for (char* t = std::strtok(argv[++i], ","); t; t = std::strtok(nullptr, ","))
+                sweep_nh.push_back(std::atoi(t));
+            continue;
+        }
+        if (std::strcmp(argv[i], "--many") == 0 && i + 1 < argc) {
+            many = std::max(1, std::atoi(argv[++i]));
+            continue;
+        }
+        if (std::strcmp(argv[i], "--results") == 0 && i + 1 < argc) {
+            results_path = argv[++i];
             continue;
         }
         args.push_back(argv[i]);
@@ -113,14 +146,8 @@ int main(int argc, char** argv) {
             status = EXIT_FAILURE;
         } else {
             const double ms = t1 - t0;
-            // the reference's naive byte count (cu:2193-2198)
-            const double hw = (nh - 1) / 2;
-            const double vox = (vs.nx - hw) * (vs.ny - hw) * (vs.nz - hw);
-            const double rB = 4.0 * vox * ((double)vs.n_atlas * nh * nh * nh +
-                                           2.0 * vs.n_atlas * 27 * nh * nh * nh);
-            const double wB = vox * 4.0;
             std::printf("Effective BandWidth = %.4f GB/s, Elapsed Time %f ms %f seconds\n",
-                        (rB + wB) / (ms * 1e-3) / 1e9, ms, ms / 1000.0);
+                        naive_gbs(vs.nx, vs.ny, vs.nz, vs.n_atlas, nh, ms), ms, ms / 1000.0);
             const std::string path = result_name + ".mat";
             rc = pct_matrixio_write(path.c_str(), out.data(), 1, vs.nx, vs.ny, vs.nz);
             if (rc) {
@@ -131,25 +158,50 @@ int main(int argc, char** argv) {
             }
         }
     } else if (choice == PCT_CHOICE_GPU_TEST_TIME) {
-        // cu:2028-2148 sweeps atlas counts for one window with 10 x 10 calls;
-        // here: every atlas count from num down to 1 for the given NH, 3
-        // samples of 3 calls of the full host wrapper (SM kernel).
-        const int many = 3;
-        for (int at = vs.n_atlas; at >= 1; --at) {
-            pb.n_atlas = at;
-            for (int i = 0; i < many && !rc; ++i) {
-                const double t0 = now_ms();
-                for (int j = 0; j < many && !rc; ++j)
-                    rc = pct_fuse_host(&pb, vs.img, vs.atlas, vs.label, out.data(), PCT_CHOICE_SM);
-                const double ms = (now_ms() - t0) / many;
-                if (!rc)
+        // cu:2028-2148: for every window edge and every atlas count from num
+        // down to 1, `many` samples of `many` calls of the host wrapper (SM);
+        // each sample's mean call time is printed, and the per-point means
+        // form the final table
+        if (sweep_nh.empty()) sweep_nh.push_back(nh);
+        std::vector<std::vector<double>> table;   // rows: nh, atlas, mean ms
+        for (int snh : sweep_nh) {
+            pb.nh = snh;
+            for (int at = vs.n_atlas; at >= 1 && !rc; --at) {
+                pb.n_atlas = at;
+                double sum = 0.0;
+                for (int i = 0; i < many && !rc; ++i) {
+                    const double t0 = now_ms();
+                    for (int j = 0; j < many && !rc; ++j)
+                        rc = pct_fuse_host(&pb, vs.img, vs.atlas, vs.label, out.data(),
+                                           PCT_CHOICE_SM);
+                    const double ms = (now_ms() - t0) / many;
+                    if (rc) break;
+                    sum += ms;
                     std::printf("Iter %d Atlas %d Choice 2 Nh %d Elapsed Time %f ms %f minutes\n",
-                                i, at, nh, ms, ms / 60000.0);
+                                i, at, snh, ms, ms / 60000.0);
+                }
+                if (rc) break;
+                const double mean = sum / many;
+                std::printf("Effective BandWidth = %.4f GB/s, Elapsed Time %f ms %f seconds\n",
+                            naive_gbs(vs.nx, vs.ny, vs.nz, at, snh, mean), mean, mean / 1000.0);
+                table.push_back({(double)snh, (double)at, mean});
             }
-            if (rc) {
-                std::fprintf(stderr, "fusion failed: %s\n", pct_strerror(rc));
+        }
+        if (rc) {
+            std::fprintf(stderr, "fusion failed: %s\n", pct_strerror(rc));
+            status = EXIT_FAILURE;
+        } else if (!results_path.empty()) {
+            FILE* f = std::fopen(results_path.c_str(), "a");
+            if (!f) {
+                std::fprintf(stderr, "cannot write %s\n", results_path.c_str());
                 status = EXIT_FAILURE;
-                break;
+            } else {
+                std::fprintf(f, "Num of executions to calculate a result %d sampled %d times\n",
+                             many, many);
+                for (const auto& r : table)
+                    std::fprintf(f, "Nh\t%d Atlas\t%d Elapsed Time\t%g\t%g\t%g\n", (int)r[0],
+                                 (int)r[1], r[2], r[2] / 1000.0, r[2] / 60000.0);
+                std::fclose(f);
             }
         }
     } else {

@@ -252,3 +252,24 @@ def test_cli_end_to_end(gpu, tmp_path):
     compare(gpu_fused, fused)
     _final_hu_agrees(gpu_fused, fused, ref_hu, nh)
     assert np.array_equal(got, pct.postprocess(gpu_fused.copy(), nh))
+
+
+def test_cli_gpu_test_time_sweep(gpu, tmp_path):
+    # EXEC_CHOICE 4 over a window list: every (NH, atlas count) point is timed,
+    # with the reference's bandwidth line, and the averaged table is written
+    cfg = str(tmp_path / "rand.cfg")
+    with open(cfg, "w") as f:
+        f.write("1\n3\n20\n22\n18\n")          # random mode: num nx ny nz
+    cli = os.path.join(os.path.dirname(_lib.LIB_PATH), "pct_cli")
+    table = str(tmp_path / "final.txt")
+    r = subprocess.run([cli, cfg, str(tmp_path / "r"), "4", "7", "2.5", "1", "0", "--sweep-nh",
+                        "3,5", "--many", "2", "--results", table, "--seed", "4"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-3000:], r.stderr[-1000:])
+    assert r.returncode == 0
+    assert r.stdout.count("Effective BandWidth") == 2 * 3
+    assert r.stdout.count("Elapsed Time") == 2 * 3 * 2 + 2 * 3
+    lines = open(table).read().splitlines()
+    assert lines[0] == "Num of executions to calculate a result 2 sampled 2 times"
+    points = [(int(l.split()[1]), int(l.split()[3])) for l in lines[1:]]
+    assert points == [(3, 3), (3, 2), (3, 1), (5, 3), (5, 2), (5, 1)]
