@@ -335,6 +335,21 @@ int pct_read_config(const char* path, uint64_t seed, pct_volume_set* out);
 void pct_volume_set_free(pct_volume_set* vs);
 void pct_free(void* p);
 
+/* MAT-file (level 5) inputs of the reference's MATLAB build (ReadMatFile,
+ * FindDimension, cpp:584-801) without libmat: struct `atlas` with fields
+ * MRI1..MRIn, struct `atlas_label` with as many fields, and the target as the
+ * remaining numeric variable, all nx x ny x nz; MATLAB's column-major order
+ * is converted to the library layout (cpp:377-474).  Numeric classes are
+ * converted to float; zlib-compressed elements are read.  Little-endian
+ * files only.  PCT_ERR_FORMAT for anything else, PCT_ERR_DIMS when the
+ * shapes or atlas counts differ.  Release with pct_volume_set_free. */
+int pct_mat_read(const char* path, pct_volume_set* out);
+
+/* Writes `data` (library layout) as a single-precision nx x ny x nz MATLAB
+ * array named `name` to a MAT-file, as WriteOutputMatFile (cpp:835-906). */
+int pct_mat_write(const char* path, const char* name, const float* data, int nx, int ny,
+                  int nz);
+
 /* Seeded uniform data of the reference's random mode (cpp:961-973):
  * every value 10*U[0,1). */
 int pct_synth_uniform(int nx, int ny, int nz, int n_atlas, uint64_t seed,

@@ -6,7 +6,8 @@ include/pct/pct.h); this package is the Python host interface over it.
 from .api import (CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, autotune,  # noqa: F401
                   autotune_batch, fuse_batch, set_batch_config,
                   device_count, force_tile_config, fuse_device, fused_to_hu_raw,
-                  matrixio_read, matrixio_write, multi_patch_based_segmentation_gpu,
+                  mat_read, mat_write, matrixio_read, matrixio_write,
+                  multi_patch_based_segmentation_gpu,
                   normalize_labels, normalize_labels_device, postprocess, postprocess_device,
                   problem, pseudo_ct, regularization_step_device,
                   read_config, regularization_step, round_matrix, synth_tissue, synth_uniform,
@@ -15,7 +16,8 @@ from .api import (CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, autotune,  # noqa:
 __all__ = [
     "CHOICE_GM", "CHOICE_GM2", "CHOICE_SM", "PctError", "autotune", "autotune_batch",
     "fuse_batch", "set_batch_config", "device_count",
-    "force_tile_config", "fuse_device", "fused_to_hu_raw", "matrixio_read", "matrixio_write",
+    "force_tile_config", "fuse_device", "fused_to_hu_raw", "mat_read", "mat_write",
+    "matrixio_read", "matrixio_write",
     "multi_patch_based_segmentation_gpu", "normalize_labels", "normalize_labels_device",
     "postprocess", "postprocess_device", "problem", "pseudo_ct", "read_config",
     "regularization_step", "regularization_step_device", "round_matrix",

@@ -124,6 +124,8 @@ SIGNATURES = {
     "pct_matrixio_write": (c_int, [c_char_p, FP, c_int, c_int, c_int, c_int]),
     "pct_read_config": (c_int, [c_char_p, c_uint64, POINTER(VolumeSet)]),
     "pct_volume_set_free": (None, [POINTER(VolumeSet)]),
+    "pct_mat_read": (c_int, [c_char_p, POINTER(VolumeSet)]),
+    "pct_mat_write": (c_int, [c_char_p, c_char_p, FP, c_int, c_int, c_int]),
     "pct_free": (None, [c_void_p]),
     "pct_synth_uniform": (c_int, [c_int, c_int, c_int, c_int, c_uint64, FP, FP, FP]),
     "pct_synth_tissue": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_uint64, FP, FP,

@@ -280,9 +280,32 @@ def matrixio_write(path: str, arr, num: int, rows: int, cols: int, depth: int) -
 
 def read_config(path: str, seed: int = 0) -> dict:
     """readMatFile (cpp:908-1018): returns img (nz,nx,ny), atlas and label
-    (n,nz,nx,ny) with the label still in HU."""
+    (n,nz,nx,ny) with the label still in HU.  A path ending in .mat is read
+    as a MAT-file (mat_read)."""
     vs = _lib.VolumeSet()
     check(lib().pct_read_config(path.encode(), int(seed), byref(vs)), f"read_config {path}")
+    return _take_volume_set(vs)
+
+
+def mat_read(path: str) -> dict:
+    """ReadMatFile of the MATLAB build (cpp:584-704) without libmat: the
+    structs atlas / atlas_label and the target image of a MAT-file, in the
+    library layout (img (nz,nx,ny), atlas and label (n,nz,nx,ny))."""
+    vs = _lib.VolumeSet()
+    check(lib().pct_mat_read(path.encode(), byref(vs)), f"mat_read {path}")
+    return _take_volume_set(vs)
+
+
+def mat_write(path: str, name: str, arr) -> None:
+    """WriteOutputMatFile (cpp:848-906): a (nz,nx,ny) float32 volume as the
+    nx x ny x nz single-precision MATLAB array `name`."""
+    arr = _f32(arr, "arr")
+    nz, nx, ny = arr.shape
+    check(lib().pct_mat_write(path.encode(), name.encode(), fptr(arr), nx, ny, nz),
+          f"mat_write {path}")
+
+
+def _take_volume_set(vs) -> dict:
     try:
         nz, nx, ny, n = vs.nz, vs.nx, vs.ny, vs.n_atlas
         N = nx * ny * nz

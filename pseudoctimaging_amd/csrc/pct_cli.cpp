@@ -5,9 +5,11 @@
 //
 //   data         configuration file (cpp:908-960): random flag, then either
 //                num/nx/ny/nz (seeded uniform data) or atlas/label/image
-//                MatrixIO paths
+//                MatrixIO paths; or a .mat file (the MATLAB build's input:
+//                structs atlas, atlas_label and the image)
 //   result       output name; the post-processed pseudo-CT (HU) is written to
-//                <result>.mat in the MatrixIO text format, as the reference
+//                <result>.mat: a MAT-file when the data was one (the MATLAB
+//                build), else the MatrixIO text format, as the reference
 //                CPU program's non-MATLAB build does (cpp:1741-1748)
 //   EXEC_CHOICE  0 GM, 1 GM2 (direct kernel), 2 SM (tiled kernel),
 //                4 GpuTestTime (timing sweep, cu:2028-2148)
@@ -101,6 +103,12 @@ int main(int argc, char** argv) {
         beta = std::atof(args[6]);
     }
     const std::string result_name = args[2];
+    const std::string data_path = args[1];
+    const bool mat_input =
+        data_path.size() > 4 && data_path.compare(data_path.size() - 4, 4, ".mat") == 0;
+    // a MATLAB variable name: the result name's last path component
+    std::string result_var = result_name.substr(result_name.find_last_of('/') + 1);
+    if (result_var.empty()) result_var = "result";
     const int choice = std::atoi(args[3]);
     const int nh = std::atoi(args[4]);
 
@@ -149,7 +157,14 @@ int main(int argc, char** argv) {
             std::printf("Effective BandWidth = %.4f GB/s, Elapsed Time %f ms %f seconds\n",
                         naive_gbs(vs.nx, vs.ny, vs.nz, vs.n_atlas, nh, ms), ms, ms / 1000.0);
             const std::string path = result_name + ".mat";
-            rc = pct_matrixio_write(path.c_str(), out.data(), 1, vs.nx, vs.ny, vs.nz);
+            // the MATLAB build (data from a .mat file) writes a MAT-file with
+            // the result name as variable (cpp:1743-1745); the text build
+            // writes MatrixIO text under the same name (cpp:1747)
+            if (mat_input)
+                rc = pct_mat_write(path.c_str(), result_var.c_str(), out.data(), vs.nx, vs.ny,
+                                   vs.nz);
+            else
+                rc = pct_matrixio_write(path.c_str(), out.data(), 1, vs.nx, vs.ny, vs.nz);
             if (rc) {
                 std::fprintf(stderr, "cannot write %s: %s\n", path.c_str(), pct_strerror(rc));
                 status = EXIT_FAILURE;

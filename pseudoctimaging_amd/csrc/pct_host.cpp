@@ -372,6 +372,9 @@ extern "C" int pct_synth_tissue(int nx, int ny, int nz, int n_atlas, int max_shi
 extern "C" int pct_read_config(const char* path, uint64_t seed, pct_volume_set* out) {
     if (!path || !out) return PCT_ERR_INVALID;
     std::memset(out, 0, sizeof(*out));
+    // the MATLAB build takes a .mat file as its data argument (cpp:1513-1519)
+    const size_t len = std::strlen(path);
+    if (len > 4 && std::strcmp(path + len - 4, ".mat") == 0) return pct_mat_read(path, out);
     std::ifstream in(path);
     if (!in.is_open()) return PCT_ERR_IO;
     // cpp:922-958: a line-indexed state machine

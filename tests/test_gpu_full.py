@@ -273,3 +273,32 @@ def test_cli_gpu_test_time_sweep(gpu, tmp_path):
     assert lines[0] == "Num of executions to calculate a result 2 sampled 2 times"
     points = [(int(l.split()[1]), int(l.split()[3])) for l in lines[1:]]
     assert points == [(3, 3), (3, 2), (3, 1), (5, 3), (5, 2), (5, 1)]
+
+
+def test_cli_mat_file_round_trip(gpu, tmp_path):
+    # the MATLAB build's flow: inputs from a MAT-file, the HU result written
+    # as a MAT-file variable named after the result
+    import scipy.io
+    nx, ny, nz, n, nh = 24, 30, 22, 2, 5
+    img, amri, act = pct.synth_tissue(nx, ny, nz, n, 2, False, 51)
+
+    def matlab(v):                                   # library (z, x, y) -> MATLAB (x, y, z)
+        return np.ascontiguousarray(np.transpose(v, (1, 2, 0)))
+    data = str(tmp_path / "subject.mat")
+    scipy.io.savemat(data, {"img": matlab(img),
+                            "atlas": {f"MRI{k + 1}": matlab(amri[k]) for k in range(n)},
+                            "atlas_label": {f"MRI{k + 1}": matlab(act[k]) for k in range(n)}},
+                     do_compression=True)
+    cli = os.path.join(os.path.dirname(_lib.LIB_PATH), "pct_cli")
+    res = str(tmp_path / "pseudo_ct")
+    r = subprocess.run([cli, data, res, "2", str(nh), "2.5", "1", "0"], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0
+    got = scipy.io.loadmat(res + ".mat")["pseudo_ct"]
+    assert got.dtype == np.float32 and got.shape == (nx, ny, nz)
+    lab = act.copy()
+    pct.normalize_labels(lab)
+    fused = pct.multi_patch_based_segmentation_gpu(img, amri, lab, nh, 2.5, 1.0)
+    want = pct.postprocess(fused.copy(), nh)
+    assert np.array_equal(np.transpose(got, (2, 0, 1)), want)
