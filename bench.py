@@ -104,7 +104,8 @@ def batched_leg(steps: int, warmup: int, seed: int):
         vox += nx * ny * nz
         cvp += interior(nx, r) * interior(ny, r) * interior(nz, r) * n * nh ** 3
         del img, amri, act, lab
-    tuned = pct.autotune_batch(subs, reps=1)
+    tuned = pct.autotune_batch(subs, reps=2)
+    choice = pct.batch_choice(subs)
     for _ in range(warmup):
         pct.fuse_batch(subs)
     torch.cuda.synchronize()
@@ -114,10 +115,28 @@ def batched_leg(steps: int, warmup: int, seed: int):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     finite = all(bool(torch.isfinite(s["out"]).all()) for s in subs)
+    # the same subjects one call each, every call with its own tuned configuration
+    seq = 0.0
+    for s in subs:
+        def one(s=s):
+            pct.fuse_device(s["img"], s["atlas"], s["label"], s["out"], s["nh"], 2.5, 1.0)
+        pct.autotune(s["img"], s["atlas"], s["label"], s["out"], s["nh"], 2.5, 1.0, reps=1)
+        one()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            one()
+        torch.cuda.synchronize()
+        seq += (time.perf_counter() - t1) / steps
     return {"subjects": [list(b) for b in BATCH], "ms_per_batch": dt * 1e3,
             "voxels_per_s": vox / dt, "cvp_per_s": cvp / dt,
             "achieved_tflops_86_per_cvp": FLOPS_PER_CVP * cvp / dt / 1e12,
-            "config": tuned, "outputs_finite": finite, "launch": "one hipGraph per batch"}
+            "config": tuned,
+            "per_class": {f"NH{2 * hw + 1}": {"shape": sid, "atlas_group": grp}
+                          for hw, (sid, grp) in choice["classes"].items()},
+            "classes_serial": choice["serial"],
+            "sequential_tuned_ms": seq * 1e3,
+            "outputs_finite": finite, "launch": "one hipGraph per batch"}
 
 
 def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup, seed,
@@ -344,7 +363,8 @@ def main() -> int:
         log("batched leg: six uneven subjects in one graph launch")
         batched = batched_leg(max(2, args.steps // 2), 1, args.seed)
         log(f"batched: {batched['ms_per_batch']:.2f} ms per batch, "
-            f"{batched['voxels_per_s']:.4e} voxels/s")
+            f"{batched['voxels_per_s']:.4e} voxels/s; one call per subject "
+            f"{batched['sequential_tuned_ms']:.2f} ms")
 
     cpu = None
     if not args.no_cpu_baseline:

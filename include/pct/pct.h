@@ -171,15 +171,36 @@ size_t pct_fuse_batch_workspace_size(const pct_subject* subjects, int n_subjects
 int pct_fuse_batch_dev_ws(const pct_subject* subjects, int n_subjects, void* d_workspace,
                           size_t workspace_bytes, pct_stream_t stream);
 
-/* Measures every batched tile shape and atlas group on these subjects,
- * keeps the fastest for this set of problem shapes.  Synchronises. */
+/* Measures, for every window class of these subjects, the batched tile
+ * shapes and atlas groups on that class alone, then the winners with the
+ * classes concurrent and one after another; keeps the fastest for this set
+ * of problem shapes.  `best` reports the configuration of the class that
+ * took longest and the measured time of the whole batch.  Synchronises. */
 int pct_autotune_batch(const pct_subject* subjects, int n_subjects, int reps,
                        pct_stream_t stream, pct_tile_config* best);
 
-/* Sets the tile shape (pct_tile_config.id) and atlas group used for this set
- * of problem shapes. */
+/* Sets one tile shape (pct_tile_config.id) and atlas group for every window
+ * class of this set of problem shapes, the classes running concurrently. */
 int pct_fuse_batch_config(const pct_subject* subjects, int n_subjects, int shape_id,
                           int atlas_group);
+
+/* The configuration of a batch per window class, indexed by the half window
+ * (NH-1)/2 = 1..7 (index 0 unused). */
+#define PCT_BATCH_CLASSES 8
+typedef struct pct_batch_choice {
+    int shape_id[PCT_BATCH_CLASSES];     /* -1: no tiled subject of that class */
+    int atlas_group[PCT_BATCH_CLASSES];
+    int serial;                          /* 1: class kernels one after another; 0: concurrent */
+} pct_batch_choice;
+
+/* The configuration pct_fuse_batch_dev uses for these subjects (tuned, set
+ * or the cost model's). */
+int pct_fuse_batch_choice_get(const pct_subject* subjects, int n_subjects,
+                              pct_batch_choice* choice);
+
+/* Sets the configuration of every window class present in `subjects`. */
+int pct_fuse_batch_choice_set(const pct_subject* subjects, int n_subjects,
+                              const pct_batch_choice* choice);
 
 /* ---- learned candidate proposal (no counterpart in the reference, which
  * always searches all n*NH^3 candidates, cpp:1399-1432; for callers that

@@ -4,7 +4,8 @@ The fusion runs in hand-written HIP kernels inside libpct.so (C ABI in
 include/pct/pct.h); this package is the Python host interface over it.
 """
 from .api import (CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, autotune,  # noqa: F401
-                  autotune_batch, fuse_batch, set_batch_config,
+                  autotune_batch, batch_choice, fuse_batch, set_batch_choice,
+                  set_batch_config,
                   device_count, force_tile_config, fuse_device, fused_to_hu_raw,
                   mat_read, mat_write, matrixio_read, matrixio_write,
                   multi_patch_based_segmentation_gpu,
@@ -15,7 +16,7 @@ from .api import (CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, autotune,  # noqa:
 
 __all__ = [
     "CHOICE_GM", "CHOICE_GM2", "CHOICE_SM", "PctError", "autotune", "autotune_batch",
-    "fuse_batch", "set_batch_config", "device_count",
+    "batch_choice", "fuse_batch", "set_batch_choice", "set_batch_config", "device_count",
     "force_tile_config", "fuse_device", "fused_to_hu_raw", "mat_read", "mat_write",
     "matrixio_read", "matrixio_write",
     "multi_patch_based_segmentation_gpu", "normalize_labels", "normalize_labels_device",

@@ -54,6 +54,13 @@ class Subject(Structure):
 
 
 PCT_MAX_BATCH = 16
+PCT_BATCH_CLASSES = 8
+
+
+class BatchChoice(Structure):
+    """pct_batch_choice: tile shape and atlas group per half window (NH-1)/2"""
+    _fields_ = [("shape_id", c_int * PCT_BATCH_CLASSES), ("atlas_group", c_int * PCT_BATCH_CLASSES),
+                ("serial", c_int)]
 
 
 class VolumeSet(Structure):
@@ -103,6 +110,8 @@ SIGNATURES = {
     "pct_autotune_batch": (c_int, [POINTER(Subject), c_int, c_int, c_void_p,
                                    POINTER(TileConfig)]),
     "pct_fuse_batch_config": (c_int, [POINTER(Subject), c_int, c_int, c_int]),
+    "pct_fuse_batch_choice_get": (c_int, [POINTER(Subject), c_int, POINTER(BatchChoice)]),
+    "pct_fuse_batch_choice_set": (c_int, [POINTER(Subject), c_int, POINTER(BatchChoice)]),
     "pct_mlp_init": (c_int, [FP, c_uint64, c_float, c_float]),
     "pct_embed_dev": (c_int, [FP, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
     "pct_fuse_pruned_dev": (c_int, [POINTER(Problem), FP, POINTER(PrunedOpts), c_void_p, c_void_p,

@@ -187,6 +187,29 @@ def set_batch_config(subjects, config_id: int, atlas_group: int) -> None:
           "pct_fuse_batch_config")
 
 
+def batch_choice(subjects) -> dict:
+    """The configuration fuse_batch uses for these subjects:
+    {"classes": {half_window: (shape_id, atlas_group)}, "serial": bool}."""
+    arr = _subjects(subjects)
+    ch = _lib.BatchChoice()
+    check(lib().pct_fuse_batch_choice_get(arr, len(subjects), byref(ch)),
+          "pct_fuse_batch_choice_get")
+    classes = {hw: (ch.shape_id[hw], ch.atlas_group[hw])
+               for hw in range(1, _lib.PCT_BATCH_CLASSES) if ch.shape_id[hw] >= 0}
+    return {"classes": classes, "serial": bool(ch.serial)}
+
+
+def set_batch_choice(subjects, classes: dict, serial: bool = False) -> None:
+    """Sets the tile shape and atlas group per half window (NH-1)/2."""
+    arr = _subjects(subjects)
+    ch = _lib.BatchChoice()
+    for hw in range(_lib.PCT_BATCH_CLASSES):
+        ch.shape_id[hw], ch.atlas_group[hw] = classes.get(hw, (-1, 1))
+    ch.serial = int(bool(serial))
+    check(lib().pct_fuse_batch_choice_set(arr, len(subjects), byref(ch)),
+          "pct_fuse_batch_choice_set")
+
+
 # ---- post-processing --------------------------------------------------------
 
 def regularization_step(arr: np.ndarray, nh: int) -> int:

@@ -1,20 +1,28 @@
 // Batched fusion: many problems (subjects) of uneven volume size, atlas
-// count and search window in one persistent kernel.
+// count and search window in one graph-captured launch of persistent kernels.
 //
 // The work items of the subjects - (tile, atlas group) pairs of the tiled
-// kernel - form one queue per search-window class (half window HW), subjects
-// with the longest items first.  For every class present, a persistent grid
-// of as many workgroups as the GPU holds at once pulls items from its queue
-// with a device-scope atomic counter until the queue is empty; each item runs
-// the same tiled_item body as the single-problem kernel, fully specialised
-// on HW (one kernel per class keeps the register allocation of that window
-// instead of the union over all windows).  A subject's result is therefore
-// bit-identical to its individual run with the same tile shape and atlas
-// group.  The class kernels are concurrent branches of one captured hipGraph,
-// the longest class first, so the tail of one queue overlaps the next; the
-// counter reset, the direct kernel for windows without a tiled
-// specialisation and the per-subject finalize passes complete the graph,
-// which is replayed while the subject descriptors stay the same.
+// kernel - are grouped by search-window class (half window HW).  Every class
+// has its own tile shape and atlas group and one persistent kernel, fully
+// specialised on HW (the register allocation of that window, not the union
+// over all windows), with as many workgroups as the GPU holds at once.
+//
+// Queues follow the XCD split of the single-problem kernel: each subject's
+// items are cut into eight contiguous parts, and chunk x of a class queue is
+// the x-th part of every subject of the class, longest items first.  A
+// workgroup starts on the chunk of its own XCD (workgroups are dispatched
+// round-robin over the eight XCDs), so tiles that share halos share an L2,
+// and moves on to the other chunks once its own is drained.  Each chunk has
+// a device-scope counter; every workgroup leaves after it has found all
+// eight chunks drained.  A subject's result is bit-identical to its
+// individual run with the same tile shape and atlas group.
+//
+// The class kernels run either concurrently, as branches of the graph, so
+// the tail of one class overlaps the next, or one after another; the
+// autotuner measures both.  The counter reset, the direct kernel for windows
+// without a tiled specialisation and the per-subject finalize passes
+// complete the graph, which is replayed while the subject descriptors stay
+// the same.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,28 +37,51 @@
 
 namespace pct {
 
+constexpr int kChunks = 8;                  // one queue chunk per XCD
+constexpr size_t kQueueBytes = 512;         // counters at the workspace start
+static_assert(kMaxTiledHalfWindow * kChunks * 8 <= (int)kQueueBytes, "queue counters");
+static_assert(PCT_BATCH_CLASSES == kMaxTiledHalfWindow + 1, "class index is the half window");
+
 struct BatchArgs {
     TiledArgs sub[PCT_MAX_BATCH];            // subjects of one window class, in queue order
-    long long begin[PCT_MAX_BATCH + 1];      // first queue position of each subject
+    long long n_items[PCT_MAX_BATCH];
     int n_sub;
-    unsigned long long* counter;
+    unsigned long long* counters;            // kChunks claim counters of the class
 };
 static_assert(sizeof(BatchArgs) <= 4096, "batch descriptor must fit the kernel arguments");
+
+// Part x of n items split into eight contiguous parts (the split of
+// xcd_contiguous): its length and first item.
+__device__ __forceinline__ long long part_len(long long n, int x) {
+    return (n >> 3) + ((n & 7) > x ? 1 : 0);
+}
+__device__ __forceinline__ long long part_start(long long n, int x) {
+    const long long per = n >> 3, rem = n & 7;
+    return x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+}
 
 template <int W, int BX, int BZ, int NW, int HW>
 __global__ __launch_bounds__(NW * 64) void fuse_batched(BatchArgs args) {
     extern __shared__ float lds[];
-    __shared__ long long next_item;
-    const long long total = args.begin[args.n_sub];
-    for (;;) {
-        if (threadIdx.x == 0) next_item = (long long)atomicAdd(args.counter, 1ULL);
-        __syncthreads();
-        const long long it = next_item;
-        __syncthreads();
-        if (it >= total) break;   // every workgroup leaves once the queue is drained
-        int k = 0;
-        while (k + 1 < args.n_sub && args.begin[k + 1] <= it) ++k;
-        tiled_item<W, BX, BZ, NW, HW>(args.sub[k], (int)(it - args.begin[k]), lds);
+    __shared__ long long claimed;
+    const int home = blockIdx.x & (kChunks - 1);
+    for (int c = 0; c < kChunks; ++c) {
+        const int x = (home + c) & (kChunks - 1);
+        for (;;) {
+            if (threadIdx.x == 0) claimed = (long long)atomicAdd(args.counters + x, 1ULL);
+            __syncthreads();
+            long long pos = claimed;
+            __syncthreads();
+            int k = 0;
+            for (; k < args.n_sub; ++k) {
+                const long long len = part_len(args.n_items[k], x);
+                if (pos < len) break;
+                pos -= len;
+            }
+            if (k == args.n_sub) break;   // chunk x is drained: every later claim is too
+            tiled_item<W, BX, BZ, NW, HW>(
+                args.sub[k], (int)(part_start(args.n_items[k], x) + pos), lds);
+        }
     }
 }
 
@@ -79,7 +110,9 @@ BatchKernel make_batch_kernel(int shape_id) {
 // Tile shapes with persistent instantiations (ids of shapes()).
 static const std::vector<BatchKernel>& batch_kernels() {
     static const std::vector<BatchKernel> k = {
+        make_batch_kernel<32, 2, 4, 4>(7),
         make_batch_kernel<32, 4, 2, 4>(8),
+        make_batch_kernel<64, 2, 4, 4>(9),
         make_batch_kernel<32, 2, 2, 8>(10),
         make_batch_kernel<32, 2, 2, 4>(11),
         make_batch_kernel<32, 2, 4, 8>(13),
@@ -98,42 +131,60 @@ static bool tiled_window(const pct_problem& pb) {
     return hw >= 1 && hw <= kMaxTiledHalfWindow;
 }
 
+// A tiled subject has work items when its interior (border r = HW + 1 on
+// every side) is not empty; that does not depend on the tile shape.
+static bool has_items(const pct_problem& pb) {
+    if (!tiled_window(pb)) return false;
+    const int r = (pb.nh - 1) / 2 + 1;
+    return pb.nx > 2 * r && pb.ny > 2 * r && pb.nz > 2 * r;
+}
+
+static int class_of(const pct_problem& pb) { return (pb.nh - 1) / 2; }
+
 // Layout of a batch in the workspace and in the queues for one configuration.
 struct BatchPlan {
-    int shape_id = -1, group = 1;
+    pct_batch_choice choice;
     int n = 0;
     std::vector<TileGeometry> geo;            // per subject (input order)
     std::vector<size_t> ws_off;               // per subject, bytes from the workspace start
-    std::vector<std::vector<int>> classes;    // tiled subjects by window class, longest first
+    std::vector<std::vector<int>> classes;    // subjects with items by window class
     std::vector<int> class_hw;
     std::vector<int> class_lds;
     size_t ws_bytes = 0;
 };
 
-static bool plan_batch(const pct_subject* s, int n, int shape_id, int group, BatchPlan* p) {
-    p->shape_id = shape_id;
-    p->group = group;
+static bool plan_batch(const pct_subject* s, int n, const pct_batch_choice& ch, BatchPlan* p) {
+    p->choice = ch;
     p->n = n;
     p->geo.assign(n, TileGeometry());
     p->ws_off.assign(n, 0);
     p->classes.clear();
     p->class_hw.clear();
     p->class_lds.clear();
-    size_t off = 256;   // the queue counters, one per window class
+    size_t off = kQueueBytes;
     std::vector<double> item_cost(n, 0.0);
     std::vector<int> by_hw[kMaxTiledHalfWindow + 1];
     for (int i = 0; i < n; ++i) {
         if (!tiled_window(s[i].pb)) continue;
+        const int hw = class_of(s[i].pb);
+        int sid = ch.shape_id[hw], grp = ch.atlas_group[hw];
+        if (sid < 0) {
+            // a class whose subjects have no items: the geometry only sizes
+            // the finalize pass
+            if (has_items(s[i].pb)) return false;
+            sid = batch_kernels().front().shape_id;
+            grp = 1;
+        }
+        if (!find_batch_kernel(sid) || grp < 1) return false;
         TileGeometry g;
-        if (!make_geometry(&s[i].pb, shape_id, std::min(group, s[i].pb.n_atlas), &g))
-            return false;
+        if (!make_geometry(&s[i].pb, sid, std::min(grp, s[i].pb.n_atlas), &g)) return false;
         p->geo[i] = g;
         p->ws_off[i] = off;
         off += tiled_ws_bytes(&s[i].pb, g.n_groups);
         item_cost[i] = g.group;
-        if (g.n_items > 0) by_hw[g.hw].push_back(i);
+        if (g.n_items > 0) by_hw[hw].push_back(i);
     }
-    // classes with the longest items first; inside a class, longest items first
+    // classes with the widest windows first; inside a class, longest items first
     for (int hw = kMaxTiledHalfWindow; hw >= 1; --hw) {
         if (by_hw[hw].empty()) continue;
         std::stable_sort(by_hw[hw].begin(), by_hw[hw].end(),
@@ -176,44 +227,46 @@ static int persistent_grid(int dev, int shape_id, BatchedFn fn, int lds_bytes) {
     return prop.multiProcessorCount * per_cu;
 }
 
-// Enqueues one batched run on `stream`, which is being captured: the class
-// kernels fork onto `side` streams and join back before the finalize passes.
+// Enqueues one batched run on `stream`, which is being captured.  With
+// concurrent classes the class kernels fork onto `side` streams and join
+// back before the finalize passes; serial classes stay on `stream`.
 static int enqueue_batch(const pct_subject* s, const BatchPlan& p, const std::vector<int>& grids,
                          void* ws, hipStream_t stream, std::vector<hipStream_t>& side,
                          std::vector<hipEvent_t>& events) {
-    const BatchKernel* bk = find_batch_kernel(p.shape_id);
-    if (!bk) return PCT_ERR_UNSUPPORTED;
     char* base = static_cast<char*>(ws);
     unsigned long long* counters = reinterpret_cast<unsigned long long*>(base);
     const int nc = (int)p.classes.size();
-    hipLaunchKernelGGL(reset_queues, dim3(1), dim3(64), 0, stream, counters, nc);
+    const bool concurrent = !p.choice.serial && nc > 1;
+    hipLaunchKernelGGL(reset_queues, dim3(1), dim3(64), 0, stream, counters, nc * kChunks);
     if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
-    if (hipEventRecord(events[0], stream) != hipSuccess) return PCT_ERR_HIP;
+    if (concurrent && hipEventRecord(events[0], stream) != hipSuccess) return PCT_ERR_HIP;
     for (int c = 0; c < nc; ++c) {
-        hipStream_t st = c == 0 ? stream : side[c - 1];
-        if (c > 0 && hipStreamWaitEvent(st, events[0], 0) != hipSuccess) return PCT_ERR_HIP;
+        const int hw = p.class_hw[c];
+        const int sid = p.choice.shape_id[hw];
+        const BatchKernel* bk = find_batch_kernel(sid);
+        if (!bk) return PCT_ERR_UNSUPPORTED;
+        hipStream_t st = (concurrent && c > 0) ? side[c - 1] : stream;
+        if (concurrent && c > 0 && hipStreamWaitEvent(st, events[0], 0) != hipSuccess)
+            return PCT_ERR_HIP;
         BatchArgs a;
         std::memset(&a, 0, sizeof(a));
-        a.counter = counters + c;
-        long long pos = 0;
+        a.counters = counters + c * kChunks;
         int k = 0;
         for (int i : p.classes[c]) {
             a.sub[k] = make_tiled_args(&s[i].pb, p.geo[i], s[i].d_img, s[i].d_atlas,
                                        s[i].d_atlas_label_norm,
                                        reinterpret_cast<float*>(base + p.ws_off[i]));
-            a.begin[k] = pos;
-            pos += p.geo[i].n_items;
+            a.n_items[k] = p.geo[i].n_items;
             ++k;
         }
-        a.begin[k] = pos;
         a.n_sub = k;
-        BatchedFn fn = bk->fn[p.class_hw[c] - 1];
+        BatchedFn fn = bk->fn[hw - 1];
         void* kargs[] = {&a};
         if (hipLaunchKernel(reinterpret_cast<const void*>(fn), dim3(grids[c]),
-                            dim3(shapes()[p.shape_id].waves * 64), kargs,
-                            (size_t)p.class_lds[c], st) != hipSuccess)
+                            dim3(shapes()[sid].waves * 64), kargs, (size_t)p.class_lds[c],
+                            st) != hipSuccess)
             return PCT_ERR_HIP;
-        if (c > 0) {
+        if (concurrent && c > 0) {
             if (hipEventRecord(events[c], st) != hipSuccess) return PCT_ERR_HIP;
             if (hipStreamWaitEvent(stream, events[c], 0) != hipSuccess) return PCT_ERR_HIP;
         }
@@ -245,7 +298,7 @@ struct CachedGraph {
 };
 
 static std::mutex g_batch_mutex;
-static std::map<std::vector<char>, std::pair<int, int>> g_batch_tuned;   // shapes -> config
+static std::map<std::vector<char>, pct_batch_choice> g_batch_tuned;   // shapes -> choice
 static std::map<BatchKey, CachedGraph> g_graphs;
 
 static std::vector<char> shape_signature(const pct_subject* s, int n) {
@@ -257,21 +310,74 @@ static std::vector<char> shape_signature(const pct_subject* s, int n) {
     return v;
 }
 
-// Model choice: the shape and group with the least summed single-problem
-// cost estimate (the batch shares one shape and one group).
-static bool choose_batch(const pct_subject* s, int n, int* shape_id, int* group) {
+static pct_batch_choice empty_choice() {
+    pct_batch_choice c;
+    for (int h = 0; h < PCT_BATCH_CLASSES; ++h) {
+        c.shape_id[h] = -1;
+        c.atlas_group[h] = 1;
+    }
+    c.serial = 0;
+    return c;
+}
+
+// Candidate atlas groups of a class: 1, 2 and all atlases of its largest
+// subject.
+static std::vector<int> class_groups(const pct_subject* s, int n, int hw) {
+    int max_n = 1;
+    for (int i = 0; i < n; ++i)
+        if (has_items(s[i].pb) && class_of(s[i].pb) == hw)
+            max_n = std::max(max_n, s[i].pb.n_atlas);
+    std::vector<int> g = {1, 2, max_n};
+    std::sort(g.begin(), g.end());
+    g.erase(std::unique(g.begin(), g.end()), g.end());
+    return g;
+}
+
+// Cost model choice per class: the shape and group with the least pooled item
+// time over the subjects of the class (a persistent grid balances items
+// across subjects, so the class costs its total item time over the slots).
+// The classes run one after another: on MI355X concurrent class kernels
+// measured slower (six subjects of 0.4-21 ms: 56.0 ms concurrent, 51.4 ms
+// serial).
+static pct_batch_choice model_choice(const pct_subject* s, int n) {
+    pct_batch_choice ch = empty_choice();
+    ch.serial = 1;
+    for (int hw = 1; hw <= kMaxTiledHalfWindow; ++hw) {
+        double best = -1;
+        for (const auto& bk : batch_kernels())
+            for (int grp : class_groups(s, n, hw)) {
+                double cost = 0;
+                bool any = false, ok = true;
+                for (int i = 0; i < n && ok; ++i) {
+                    if (!has_items(s[i].pb) || class_of(s[i].pb) != hw) continue;
+                    TileGeometry g;
+                    const double t = model_item_time(&s[i].pb, bk.shape_id,
+                                                     std::min(grp, s[i].pb.n_atlas), &g);
+                    if (t < 0) {
+                        ok = false;
+                        break;
+                    }
+                    cost += t * (double)g.n_items / ((double)kNumCu * g.wg_per_cu);
+                    any = true;
+                }
+                if (!ok || !any) continue;
+                if (best < 0 || cost < best * 0.999) {
+                    best = cost;
+                    ch.shape_id[hw] = bk.shape_id;
+                    ch.atlas_group[hw] = grp;
+                }
+            }
+    }
+    return ch;
+}
+
+static pct_batch_choice choose_batch(const pct_subject* s, int n) {
     {
         std::lock_guard<std::mutex> lk(g_batch_mutex);
         auto it = g_batch_tuned.find(shape_signature(s, n));
-        if (it != g_batch_tuned.end()) {
-            *shape_id = it->second.first;
-            *group = it->second.second;
-            return true;
-        }
+        if (it != g_batch_tuned.end()) return it->second;
     }
-    *shape_id = batch_kernels().front().shape_id;
-    *group = 1;
-    return true;
+    return model_choice(s, n);
 }
 
 static void destroy_graph(CachedGraph& g) {
@@ -281,13 +387,13 @@ static void destroy_graph(CachedGraph& g) {
     g.graph = nullptr;
 }
 
-static int run_batch(const pct_subject* s, int n, int shape_id, int group, void* ws,
+static int run_batch(const pct_subject* s, int n, const pct_batch_choice& ch, void* ws,
                      size_t ws_bytes, bool own_ws, hipStream_t stream) {
     int dev = 0;
     int rc = check_device(&dev);
     if (rc) return rc;
     BatchPlan p;
-    if (!plan_batch(s, n, shape_id, group, &p)) return PCT_ERR_UNSUPPORTED;
+    if (!plan_batch(s, n, ch, &p)) return PCT_ERR_UNSUPPORTED;
     if (own_ws) {
         rc = ensure_workspace(dev, p.ws_bytes, &ws);
         if (rc) return rc;
@@ -298,8 +404,9 @@ static int run_batch(const pct_subject* s, int n, int shape_id, int group, void*
     {
         const char* b = reinterpret_cast<const char*>(s);
         key.bytes.assign(b, b + sizeof(pct_subject) * n);
-        const long long extra[4] = {shape_id, group, (long long)reinterpret_cast<uintptr_t>(ws),
-                                    dev};
+        const char* c = reinterpret_cast<const char*>(&ch);
+        key.bytes.insert(key.bytes.end(), c, c + sizeof(ch));
+        const long long extra[2] = {(long long)reinterpret_cast<uintptr_t>(ws), dev};
         const char* e = reinterpret_cast<const char*>(extra);
         key.bytes.insert(key.bytes.end(), e, e + sizeof(extra));
     }
@@ -310,10 +417,11 @@ static int run_batch(const pct_subject* s, int n, int shape_id, int group, void*
         // captured), replay on the caller's stream
         const int nc = (int)p.classes.size();
         std::vector<int> grids(nc, 0);
-        const BatchKernel* bk = find_batch_kernel(p.shape_id);
-        if (!bk) return PCT_ERR_UNSUPPORTED;
         for (int c = 0; c < nc; ++c) {
-            grids[c] = persistent_grid(dev, p.shape_id, bk->fn[p.class_hw[c] - 1], p.class_lds[c]);
+            const int sid = p.choice.shape_id[p.class_hw[c]];
+            const BatchKernel* bk = find_batch_kernel(sid);
+            if (!bk) return PCT_ERR_UNSUPPORTED;
+            grids[c] = persistent_grid(dev, sid, bk->fn[p.class_hw[c] - 1], p.class_lds[c]);
             if (grids[c] < 1) return PCT_ERR_HIP;
         }
         hipStream_t cap = nullptr;
@@ -359,16 +467,30 @@ static int run_batch(const pct_subject* s, int n, int shape_id, int group, void*
     return PCT_OK;
 }
 
+// Mean time of `reps` runs after one warm-up run (which captures the graph).
+static int time_batch(const pct_subject* s, int n, const pct_batch_choice& ch, int reps,
+                      hipStream_t st, hipEvent_t e0, hipEvent_t e1, double* ms) {
+    int rc = run_batch(s, n, ch, nullptr, 0, true, st);
+    if (rc) return rc;
+    if (hipEventRecord(e0, st) != hipSuccess) return PCT_ERR_HIP;
+    for (int i = 0; i < reps && !rc; ++i) rc = run_batch(s, n, ch, nullptr, 0, true, st);
+    if (hipEventRecord(e1, st) != hipSuccess) return PCT_ERR_HIP;
+    if (hipEventSynchronize(e1) != hipSuccess) return PCT_ERR_HIP;
+    if (rc) return rc;
+    float t = 0;
+    if (hipEventElapsedTime(&t, e0, e1) != hipSuccess) return PCT_ERR_HIP;
+    *ms = t / reps;
+    return PCT_OK;
+}
+
 }  // namespace pct
 
 using namespace pct;
 
 extern "C" size_t pct_fuse_batch_workspace_size(const pct_subject* subjects, int n_subjects) {
     if (validate_batch(subjects, n_subjects)) return 0;
-    int sid = 0, grp = 1;
-    choose_batch(subjects, n_subjects, &sid, &grp);
     BatchPlan p;
-    if (!plan_batch(subjects, n_subjects, sid, grp, &p)) return 0;
+    if (!plan_batch(subjects, n_subjects, choose_batch(subjects, n_subjects), &p)) return 0;
     return p.ws_bytes;
 }
 
@@ -376,9 +498,7 @@ extern "C" int pct_fuse_batch_dev(const pct_subject* subjects, int n_subjects,
                                   pct_stream_t stream) {
     int rc = validate_batch(subjects, n_subjects);
     if (rc) return rc;
-    int sid = 0, grp = 1;
-    choose_batch(subjects, n_subjects, &sid, &grp);
-    return run_batch(subjects, n_subjects, sid, grp, nullptr, 0, true,
+    return run_batch(subjects, n_subjects, choose_batch(subjects, n_subjects), nullptr, 0, true,
                      reinterpret_cast<hipStream_t>(stream));
 }
 
@@ -386,10 +506,39 @@ extern "C" int pct_fuse_batch_dev_ws(const pct_subject* subjects, int n_subjects
                                      size_t ws_bytes, pct_stream_t stream) {
     int rc = validate_batch(subjects, n_subjects);
     if (rc) return rc;
-    int sid = 0, grp = 1;
-    choose_batch(subjects, n_subjects, &sid, &grp);
-    return run_batch(subjects, n_subjects, sid, grp, d_ws, ws_bytes, false,
-                     reinterpret_cast<hipStream_t>(stream));
+    return run_batch(subjects, n_subjects, choose_batch(subjects, n_subjects), d_ws, ws_bytes,
+                     false, reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int pct_fuse_batch_choice_get(const pct_subject* subjects, int n_subjects,
+                                         pct_batch_choice* choice) {
+    int rc = validate_batch(subjects, n_subjects);
+    if (rc) return rc;
+    if (!choice) return PCT_ERR_INVALID;
+    *choice = choose_batch(subjects, n_subjects);
+    return PCT_OK;
+}
+
+extern "C" int pct_fuse_batch_choice_set(const pct_subject* subjects, int n_subjects,
+                                         const pct_batch_choice* choice) {
+    int rc = validate_batch(subjects, n_subjects);
+    if (rc) return rc;
+    if (!choice || (choice->serial != 0 && choice->serial != 1)) return PCT_ERR_INVALID;
+    pct_batch_choice ch = empty_choice();
+    ch.serial = choice->serial;
+    for (int i = 0; i < n_subjects; ++i) {
+        if (!has_items(subjects[i].pb)) continue;
+        const int hw = class_of(subjects[i].pb);
+        if (!find_batch_kernel(choice->shape_id[hw]) || choice->atlas_group[hw] < 1)
+            return PCT_ERR_INVALID;
+        ch.shape_id[hw] = choice->shape_id[hw];
+        ch.atlas_group[hw] = choice->atlas_group[hw];
+    }
+    BatchPlan p;
+    if (!plan_batch(subjects, n_subjects, ch, &p)) return PCT_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(g_batch_mutex);
+    g_batch_tuned[shape_signature(subjects, n_subjects)] = ch;
+    return PCT_OK;
 }
 
 extern "C" int pct_fuse_batch_config(const pct_subject* subjects, int n_subjects, int shape_id,
@@ -397,11 +546,12 @@ extern "C" int pct_fuse_batch_config(const pct_subject* subjects, int n_subjects
     int rc = validate_batch(subjects, n_subjects);
     if (rc) return rc;
     if (!find_batch_kernel(shape_id) || atlas_group < 1) return PCT_ERR_INVALID;
-    BatchPlan p;
-    if (!plan_batch(subjects, n_subjects, shape_id, atlas_group, &p)) return PCT_ERR_UNSUPPORTED;
-    std::lock_guard<std::mutex> lk(g_batch_mutex);
-    g_batch_tuned[shape_signature(subjects, n_subjects)] = {shape_id, atlas_group};
-    return PCT_OK;
+    pct_batch_choice ch = empty_choice();
+    for (int h = 1; h <= kMaxTiledHalfWindow; ++h) {
+        ch.shape_id[h] = shape_id;
+        ch.atlas_group[h] = atlas_group;
+    }
+    return pct_fuse_batch_choice_set(subjects, n_subjects, &ch);
 }
 
 extern "C" int pct_autotune_batch(const pct_subject* subjects, int n_subjects, int reps,
@@ -411,66 +561,83 @@ extern "C" int pct_autotune_batch(const pct_subject* subjects, int n_subjects, i
     int dev = 0;
     rc = check_device(&dev);
     if (rc) return rc;
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     if (reps < 1) reps = 1;
-    int max_n = 1;
-    for (int i = 0; i < n_subjects; ++i) max_n = std::max(max_n, subjects[i].pb.n_atlas);
-    std::vector<int> groups = {1, 2, max_n};
-    std::sort(groups.begin(), groups.end());
-    groups.erase(std::unique(groups.begin(), groups.end()), groups.end());
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return PCT_ERR_HIP;
-    double best_ms = -1;
-    int best_sid = -1, best_grp = 1;
-    for (const auto& bk : batch_kernels()) {
-        for (int grp : groups) {
-            BatchPlan p;
-            if (!plan_batch(subjects, n_subjects, bk.shape_id, grp, &p)) continue;
-            rc = run_batch(subjects, n_subjects, bk.shape_id, grp, nullptr, 0, true, s);  // warm-up
-            if (rc) break;
-            (void)hipEventRecord(e0, s);
-            for (int i = 0; i < reps && !rc; ++i)
-                rc = run_batch(subjects, n_subjects, bk.shape_id, grp, nullptr, 0, true, s);
-            (void)hipEventRecord(e1, s);
-            if (hipEventSynchronize(e1) != hipSuccess) rc = PCT_ERR_HIP;
-            if (rc) break;
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            ms /= reps;
-            if (best_ms < 0 || ms < best_ms) {
-                best_ms = ms;
-                best_sid = bk.shape_id;
-                best_grp = grp;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return PCT_ERR_HIP;
+    }
+    pct_batch_choice ch = empty_choice();
+    double class_ms[PCT_BATCH_CLASSES] = {0};
+    // every class alone: its subjects, every shape and group
+    for (int hw = 1; hw <= kMaxTiledHalfWindow && !rc; ++hw) {
+        std::vector<pct_subject> cls;
+        for (int i = 0; i < n_subjects; ++i)
+            if (has_items(subjects[i].pb) && class_of(subjects[i].pb) == hw)
+                cls.push_back(subjects[i]);
+        if (cls.empty()) continue;
+        double best_ms = -1;
+        for (const auto& bk : batch_kernels()) {
+            for (int grp : class_groups(cls.data(), (int)cls.size(), hw)) {
+                pct_batch_choice c = empty_choice();
+                c.shape_id[hw] = bk.shape_id;
+                c.atlas_group[hw] = grp;
+                BatchPlan p;
+                if (!plan_batch(cls.data(), (int)cls.size(), c, &p)) continue;
+                double ms = 0;
+                rc = time_batch(cls.data(), (int)cls.size(), c, reps, st, e0, e1, &ms);
+                if (rc) break;
+                if (best_ms < 0 || ms < best_ms) {
+                    best_ms = ms;
+                    ch.shape_id[hw] = bk.shape_id;
+                    ch.atlas_group[hw] = grp;
+                }
             }
+            if (rc) break;
         }
-        if (rc) break;
+        if (!rc && best_ms < 0) rc = PCT_ERR_UNSUPPORTED;
+        class_ms[hw] = best_ms;
+    }
+    // the winners together: classes concurrent, then one after another
+    double batch_ms = -1;
+    for (int serial = 0; serial <= 1 && !rc; ++serial) {
+        pct_batch_choice c = ch;
+        c.serial = serial;
+        double ms = 0;
+        rc = time_batch(subjects, n_subjects, c, reps, st, e0, e1, &ms);
+        if (!rc && (batch_ms < 0 || ms < batch_ms)) {
+            batch_ms = ms;
+            ch.serial = serial;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    if (best_sid < 0) return PCT_ERR_UNSUPPORTED;
     {
         std::lock_guard<std::mutex> lk(g_batch_mutex);
-        g_batch_tuned[shape_signature(subjects, n_subjects)] = {best_sid, best_grp};
+        g_batch_tuned[shape_signature(subjects, n_subjects)] = ch;
     }
     if (best) {
-        const TileShape& sh = shapes()[best_sid];
+        int top = -1;
+        for (int hw = 1; hw <= kMaxTiledHalfWindow; ++hw)
+            if (ch.shape_id[hw] >= 0 && (top < 0 || class_ms[hw] > class_ms[top])) top = hw;
         std::memset(best, 0, sizeof(*best));
-        best->id = best_sid;
-        best->seg_width = sh.seg_width;
-        best->bx = sh.bx;
-        best->bz = sh.bz;
-        best->waves = sh.waves;
-        best->atlas_group = best_grp;
         BatchPlan p;
-        plan_batch(subjects, n_subjects, best_sid, best_grp, &p);
-        best->lds_bytes = 0;
+        plan_batch(subjects, n_subjects, ch, &p);
         for (int l : p.class_lds) best->lds_bytes = std::max(best->lds_bytes, l);
-        long long items = 0;
-        for (int i = 0; i < n_subjects; ++i) items += p.geo[i].n_items;
-        best->items = items;
+        for (int i = 0; i < n_subjects; ++i) best->items += p.geo[i].n_items;
+        best->id = top > 0 ? ch.shape_id[top] : -1;
+        if (top > 0) {
+            const TileShape& sh = shapes()[ch.shape_id[top]];
+            best->seg_width = sh.seg_width;
+            best->bx = sh.bx;
+            best->bz = sh.bz;
+            best->waves = sh.waves;
+            best->atlas_group = ch.atlas_group[top];
+        }
         best->est_cost = -1;
-        best->measured_ms = best_ms;
+        best->measured_ms = batch_ms;
     }
     return PCT_OK;
 }

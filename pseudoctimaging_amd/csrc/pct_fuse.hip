@@ -204,8 +204,6 @@ static const TiledEntry* find_entry(int shape_id, int hw) {
     return &e;
 }
 
-constexpr int kNumCu = 256;
-
 int validate_problem(const pct_problem* pb) {
     if (!pb) return PCT_ERR_INVALID;
     if (pb->nx < 1 || pb->ny < 1 || pb->nz < 1 || pb->n_atlas < 1) return PCT_ERR_INVALID;
@@ -266,7 +264,7 @@ static double simd_efficiency(double w) {
 //   1.7 (BZ+2)(BX+2) + 1.5 (BZ+2) BX + 6.5 BZ BX
 // (squared differences, x sums, then z sums, DPP y sums, exp, accumulation
 // per output), within 5% of the compiled kernels; exp issues as four slots.
-static double model_cost(const pct_problem* pb, int shape_id, int group, TileGeometry* geo) {
+double model_item_time(const pct_problem* pb, int shape_id, int group, TileGeometry* geo) {
     TileGeometry g;
     if (!make_geometry(pb, shape_id, group, &g)) return -1;
     const TileShape& s = shapes()[shape_id];
@@ -283,7 +281,14 @@ static double model_cost(const pct_problem* pb, int shape_id, int group, TileGeo
     const double width_penalty = s.seg_width == 64 ? 1.5 : 1.0;
     const double wave_time = width_penalty * g.group * nh * (nh * nh * step + stage);
     const double w = (double)g.wg_per_cu * s.waves / 4.0;  // waves per SIMD
-    const double item_time = wave_time * std::max(1.0, w) / simd_efficiency(w);
+    if (geo) *geo = g;
+    return wave_time * std::max(1.0, w) / simd_efficiency(w);
+}
+
+static double model_cost(const pct_problem* pb, int shape_id, int group, TileGeometry* geo) {
+    TileGeometry g;
+    const double item_time = model_item_time(pb, shape_id, group, &g);
+    if (item_time < 0) return -1;
     const double rounds = std::ceil((double)g.n_items / ((double)kNumCu * g.wg_per_cu));
     if (geo) *geo = g;
     return rounds * item_time;

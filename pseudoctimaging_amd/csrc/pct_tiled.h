@@ -414,6 +414,12 @@ const std::vector<TileShape>& shapes();
 // kernel or does not fit the LDS).
 bool make_geometry(const pct_problem* pb, int shape_id, int group, TileGeometry* g);
 
+// Cost model: time of one work item of `shape_id` (relative units, the same
+// for every shape), or -1 where the shape has no geometry for the problem.
+double model_item_time(const pct_problem* pb, int shape_id, int group, TileGeometry* geo);
+
+constexpr int kNumCu = 256;
+
 // Checks that the current device is a gfx950; returns its ordinal.
 int check_device(int* dev_out);
 

@@ -25,7 +25,7 @@ SUBJECTS = [
     (10, 10, 10, 1, 11),
     (44, 40, 38, 4, 7),
 ]
-BATCH_SHAPES = [8, 10, 11, 13]
+BATCH_SHAPES = [7, 8, 9, 10, 11, 13]
 
 
 def make_subjects(specs, seed=0):
@@ -63,6 +63,40 @@ def test_batch_equals_individual_runs(gpu, shape_id, group):
         pct.force_tile_config(-1, 0)
 
 
+def _check_against_individual(host, subs, classes):
+    """Each subject equals its own tiled run with its class's shape and group."""
+    try:
+        for (img, amri, lab, nh), s in zip(host, subs):
+            sid, grp = classes.get((nh - 1) // 2, (BATCH_SHAPES[0], 1))
+            pct.force_tile_config(sid, grp)
+            single = pct.multi_patch_based_segmentation_gpu(img, amri, lab, nh, 2.5, 1.0)
+            got = s["out"].cpu().numpy()
+            assert np.array_equal(got.view(np.uint32), single.view(np.uint32)), (nh, sid, grp)
+    finally:
+        pct.force_tile_config(-1, 0)
+
+
+@pytest.mark.parametrize("serial", [False, True])
+def test_batch_per_class_configuration(gpu, serial):
+    """Window classes with different tile shapes and atlas groups, run
+    concurrently or one after another, in one graph."""
+    import torch
+    print(f"case batch per-class serial {serial}", flush=True)
+    host, subs = make_subjects(SUBJECTS, seed=50)
+    classes = {1: (11, 1), 3: (7, 2), 4: (9, 1), 5: (13, 2)}
+    pct.set_batch_choice(subs, classes, serial)
+    got = pct.batch_choice(subs)
+    assert got["serial"] == serial
+    # the NH=17 subject has no tiled class; (10, 10, 10) has no interior
+    assert got["classes"] == classes
+    for replay in range(2):
+        for s in subs:
+            s["out"].fill_(-3.0)
+        pct.fuse_batch(subs)
+        torch.cuda.synchronize()
+        _check_against_individual(host, subs, classes)
+
+
 def test_batch_against_oracle_and_graph_replays(gpu):
     import torch
     host, subs = make_subjects(SUBJECTS, seed=100)
@@ -86,8 +120,12 @@ def test_batch_autotune_and_workspace_variant(gpu):
     host, subs = make_subjects(SUBJECTS[:4], seed=7)
     cfg = pct.autotune_batch(subs, reps=1)
     assert cfg["id"] in BATCH_SHAPES and cfg["measured_ms"] > 0
+    choice = pct.batch_choice(subs)
+    assert set(choice["classes"]) == {3, 4, 5, 1}
+    assert all(sid in BATCH_SHAPES for sid, _ in choice["classes"].values())
     pct.fuse_batch(subs)
     torch.cuda.synchronize()
+    _check_against_individual(host, subs, choice["classes"])
     first = [s["out"].clone() for s in subs]
     arr = pct.api._subjects(subs)
     L = _lib.lib()
