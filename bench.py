@@ -17,6 +17,9 @@ Extra fields:
                 reference source) or the oracle port, OpenMP over z, timed on
                 a bounded crop of the same inputs and scaled to voxels/s.
   parity_sample float64 oracle vs the GPU output on sampled voxels.
+  end_to_end    host buffers in and out around the fusion and the host post
+                chain, the region the reference's GPU program times.
+  learned_pruned, batched  the learned candidate path and the uneven batch.
 """
 from __future__ import annotations
 
@@ -203,6 +206,43 @@ def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup,
                      "+ fallback + finalize, per call"}
 
 
+def end_to_end_leg(img, amri, lab, nh, sigma, beta, reps):
+    """The region the reference's GPU program times around its SM call
+    (src/GPU/PatchBasedPseudoCT.cu:2152-2188): host buffers in, per-call device
+    allocation, three host-to-device copies, the fusion, the copy back and
+    the post chain (NaN fill, round, HU) on the host."""
+    import numpy as np
+
+    import pseudoctimaging_amd as pct
+    nz, nx, ny = img.shape
+    n = amri.shape[0]
+    fused = pct.multi_patch_based_segmentation_gpu(img, amri, lab, nh, sigma, beta)   # warm-up
+    pct.postprocess(fused, nh)
+    t_fuse, t_post = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fused = pct.multi_patch_based_segmentation_gpu(img, amri, lab, nh, sigma, beta)
+        t1 = time.perf_counter()
+        pct.postprocess(fused, nh)
+        t2 = time.perf_counter()
+        t_fuse.append(t1 - t0)
+        t_post.append(t2 - t1)
+    fuse_s, post_s = float(np.median(t_fuse)), float(np.median(t_post))
+    # the reference's effective-bandwidth line (cu:2193-2196), on the whole region
+    h = (nh - 1) // 2
+    m = (nx - h) * (ny - h) * (nz - h)
+    rb = 4.0 * m * (n * nh ** 3 + 2 * n * 27 * nh ** 3)
+    wb = 4.0 * m
+    total = fuse_s + post_s
+    return {"fuse_host_ms": fuse_s * 1e3, "post_host_ms": post_s * 1e3,
+            "total_ms": total * 1e3, "voxels_per_s": nx * ny * nz / total,
+            "reference_effective_bandwidth_gbs": (rb + wb) / total / 1e9,
+            "pcie_bytes": 4 * nx * ny * nz * (2 + 2 * n) + 4 * nx * ny * nz,
+            "reps": reps, "statistic": "median",
+            "region": "pct_fuse_host (device alloc, 3 H2D, fusion, D2H, free) + host "
+                      "NaN fill, round and HU rescale"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,6 +261,8 @@ def main() -> int:
     ap.add_argument("--parity-voxels", type=int, default=2000)
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-pruned leg")
+    ap.add_argument("--no-end-to-end", action="store_true",
+                    help="skip the host-buffer end-to-end leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized float32 buffer three times (a plain copy "
                          "kernel of known size, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -347,6 +389,13 @@ def main() -> int:
                   "nan_mismatch": int(np.count_nonzero(np.isnan(got) != np.isnan(ref)))}
         log(f"parity sample: {parity}")
 
+    e2e = None
+    if not args.no_end_to_end:
+        log("end-to-end leg: host buffers, per-call device memory, host post chain")
+        e2e = end_to_end_leg(img, amri, lab, nh, sigma, beta, 3)
+        log(f"end-to-end: {e2e['total_ms']:.1f} ms (fusion call {e2e['fuse_host_ms']:.1f} ms, "
+            f"post {e2e['post_host_ms']:.1f} ms)")
+
     learned_res = None
     if not args.no_learned:
         log("learned leg: training the embedding on 4 other phantoms, then pruned fusion")
@@ -400,6 +449,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity_sample": parity,
+        "end_to_end": e2e,
         "learned_pruned": learned_res,
         "batched": batched,
     }

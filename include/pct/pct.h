@@ -331,8 +331,18 @@ int pct_postprocess_dev(int nh, int nx, int ny, int nz, float* d_arr, pct_stream
 /* MatrixIO text format (cpp:252-320): lines num, rows, cols, depth, then the
  * values in memory order separated by blanks.  Allocates *data (free with
  * pct_free); dims4 = {rows, cols, depth, num} as the reference's
- * `dimensions` vector (cpp:268-271). */
+ * `dimensions` vector (cpp:268-271).  Values parse as the reference's
+ * `iss >> float` does (correctly rounded), in parallel.  When a current
+ * binary sidecar "<path>.pctbin" exists (see below), its values are read
+ * instead of the text. */
 int pct_matrixio_read(const char* path, float** data, int dims4[4]);
+
+/* Writes "<path>.pctbin" beside the MatrixIO text file `path`: a header that
+ * records the text file's size and modification time, then the values as raw
+ * float32.  pct_matrixio_read uses it only while the text file keeps that
+ * size and time.  No counterpart in the reference, whose inputs are
+ * re-parsed on every run. */
+int pct_matrixio_write_sidecar(const char* path);
 
 /* Writes the MatrixIO text format with the reference's formatting (default
  * 6 significant digits, one blank after every value; cpp:289-320). */

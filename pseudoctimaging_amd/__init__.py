@@ -7,7 +7,7 @@ from .api import (CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, autotune,  # noqa:
                   autotune_batch, batch_choice, fuse_batch, set_batch_choice,
                   set_batch_config,
                   device_count, force_tile_config, fuse_device, fused_to_hu_raw,
-                  mat_read, mat_write, matrixio_read, matrixio_write,
+                  mat_read, mat_write, matrixio_read, matrixio_write, matrixio_write_sidecar,
                   multi_patch_based_segmentation_gpu,
                   normalize_labels, normalize_labels_device, postprocess, postprocess_device,
                   problem, pseudo_ct, regularization_step_device,
@@ -18,7 +18,7 @@ __all__ = [
     "CHOICE_GM", "CHOICE_GM2", "CHOICE_SM", "PctError", "autotune", "autotune_batch",
     "batch_choice", "fuse_batch", "set_batch_choice", "set_batch_config", "device_count",
     "force_tile_config", "fuse_device", "fused_to_hu_raw", "mat_read", "mat_write",
-    "matrixio_read", "matrixio_write",
+    "matrixio_read", "matrixio_write", "matrixio_write_sidecar",
     "multi_patch_based_segmentation_gpu", "normalize_labels", "normalize_labels_device",
     "postprocess", "postprocess_device", "problem", "pseudo_ct", "read_config",
     "regularization_step", "regularization_step_device", "round_matrix",

@@ -131,6 +131,7 @@ SIGNATURES = {
     "pct_postprocess_dev": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
     "pct_matrixio_read": (c_int, [c_char_p, POINTER(FP), POINTER(c_int)]),
     "pct_matrixio_write": (c_int, [c_char_p, FP, c_int, c_int, c_int, c_int]),
+    "pct_matrixio_write_sidecar": (c_int, [c_char_p]),
     "pct_read_config": (c_int, [c_char_p, c_uint64, POINTER(VolumeSet)]),
     "pct_volume_set_free": (None, [POINTER(VolumeSet)]),
     "pct_mat_read": (c_int, [c_char_p, POINTER(VolumeSet)]),

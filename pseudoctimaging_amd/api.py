@@ -301,6 +301,12 @@ def matrixio_write(path: str, arr, num: int, rows: int, cols: int, depth: int) -
           f"write {path}")
 
 
+def matrixio_write_sidecar(path: str) -> None:
+    """Writes the binary sidecar <path>.pctbin that matrixio_read prefers
+    while the text file is unchanged."""
+    check(lib().pct_matrixio_write_sidecar(path.encode()), f"sidecar {path}")
+
+
 def read_config(path: str, seed: int = 0) -> dict:
     """readMatFile (cpp:908-1018): returns img (nz,nx,ny), atlas and label
     (n,nz,nx,ny) with the label still in HU.  A path ending in .mat is read

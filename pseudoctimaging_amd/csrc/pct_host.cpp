@@ -1,8 +1,12 @@
 // Host-side part of libpct.so: pre/post-processing around the fusion hot
 // path, the MatrixIO text format, the configuration file of the reference
 // program, and seeded input generators.  None of this computes the fusion.
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <cerrno>
+#include <charconv>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -235,49 +239,218 @@ extern "C" int pct_postprocess(int nh, int nx, int ny, int nz, float* arr) {
 
 extern "C" void pct_free(void* p) { std::free(p); }
 
-extern "C" int pct_matrixio_read(const char* path, float** data, int dims4[4]) {
-    if (!path || !data || !dims4) return PCT_ERR_INVALID;
-    *data = nullptr;
-    std::ifstream in(path);
-    if (!in.is_open()) return PCT_ERR_IO;
-    std::string line;
+namespace {
+
+bool is_blank(char c) { return c == ' ' || c == '\n' || c == '\t' || c == '\r' || c == '\v' || c == '\f'; }
+
+// The whole file in memory, with a terminating NUL.
+int slurp(const char* path, std::vector<char>* buf) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return PCT_ERR_IO;
+    struct stat st;
+    std::vector<char>& b = *buf;
+    b.clear();
+    if (::fstat(fileno(f), &st) == 0 && S_ISREG(st.st_mode)) {
+        b.resize((size_t)st.st_size);
+        const size_t got = std::fread(b.data(), 1, b.size(), f);
+        b.resize(got);
+    }
+    char tmp[1 << 16];   // anything beyond the size at open (or a non-regular file)
+    size_t got;
+    while ((got = std::fread(tmp, 1, sizeof(tmp), f)) > 0) b.insert(b.end(), tmp, tmp + got);
+    const bool bad = std::ferror(f) != 0;
+    std::fclose(f);
+    if (bad) return PCT_ERR_IO;
+    b.push_back('\0');
+    return PCT_OK;
+}
+
+// One value token [p, e) as the reference's `iss >> float` reads it
+// (correctly rounded decimal; an optional leading '+').
+bool parse_float(const char* p, const char* e, float* v) {
+    if (p < e && *p == '+') {
+        ++p;
+        if (p < e && (*p == '+' || *p == '-')) return false;
+    }
+    const auto r = std::from_chars(p, e, *v);
+    return r.ec == std::errc() && r.ptr == e;
+}
+
+// Parses exactly `size` blank-separated values of [b, e) into `out`.  The
+// text is cut into chunks at blanks; every chunk counts its tokens, a prefix
+// sum gives its first index, and the chunks parse in parallel.
+int parse_values(const char* b, const char* e, float* out, long long size) {
+    const long long len = e - b;
+    const int chunks = (int)std::max<long long>(1, std::min<long long>(256, len >> 16));
+    std::vector<const char*> cut(chunks + 1);
+    cut[0] = b;
+    cut[chunks] = e;
+    for (int c = 1; c < chunks; ++c) {
+        const char* q = b + len * c / chunks;
+        while (q < e && !is_blank(*q)) ++q;   // never split a token
+        cut[c] = std::max(q, cut[c - 1]);
+    }
+    std::vector<long long> count(chunks + 1, 0);
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < chunks; ++c) {
+        long long k = 0;
+        bool in_tok = false;
+        for (const char* q = cut[c]; q < cut[c + 1]; ++q) {
+            const bool blank = is_blank(*q);
+            if (!blank && !in_tok) ++k;
+            in_tok = !blank;
+        }
+        count[c + 1] = k;
+    }
+    for (int c = 0; c < chunks; ++c) count[c + 1] += count[c];
+    if (count[chunks] != size) return PCT_ERR_FORMAT;
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int c = 0; c < chunks; ++c) {
+        long long idx = count[c];
+        const char* q = cut[c];
+        const char* end = cut[c + 1];
+        while (q < end) {
+            while (q < end && is_blank(*q)) ++q;
+            if (q == end) break;
+            const char* t = q;
+            while (q < end && !is_blank(*q)) ++q;
+            if (!parse_float(t, q, &out[idx++])) bad = 1;
+        }
+    }
+    return bad ? PCT_ERR_FORMAT : PCT_OK;
+}
+
+// Binary sidecar "<path>.pctbin": a header naming the text file it was made
+// from (byte size and modification time) followed by the values as raw
+// little-endian float32.  A sidecar that does not match its text file is
+// ignored.
+struct SidecarHeader {
+    char magic[8];
+    long long num, rows, cols, depth;
+    long long text_bytes;
+    long long text_mtime_ns;
+};
+constexpr char kSidecarMagic[8] = {'P', 'C', 'T', 'M', 'I', 'O', '1', '\0'};
+
+std::string sidecar_path(const char* path) { return std::string(path) + ".pctbin"; }
+
+bool text_stamp(const char* path, long long* bytes, long long* mtime_ns) {
+    struct stat st;
+    if (::stat(path, &st) != 0) return false;
+    *bytes = (long long)st.st_size;
+    *mtime_ns = (long long)st.st_mtim.tv_sec * 1000000000LL + st.st_mtim.tv_nsec;
+    return true;
+}
+
+// Reads the sidecar of `path` if it is there and current; false otherwise.
+bool read_sidecar(const char* path, float** data, int dims4[4]) {
+    long long bytes = 0, mtime = 0;
+    if (!text_stamp(path, &bytes, &mtime)) return false;
+    FILE* f = std::fopen(sidecar_path(path).c_str(), "rb");
+    if (!f) return false;
+    SidecarHeader h;
+    bool ok = std::fread(&h, sizeof(h), 1, f) == 1 &&
+              std::memcmp(h.magic, kSidecarMagic, sizeof(kSidecarMagic)) == 0 &&
+              h.text_bytes == bytes && h.text_mtime_ns == mtime && h.num > 0 && h.rows > 0 &&
+              h.cols > 0 && h.depth > 0 && h.num <= INT_MAX && h.rows <= INT_MAX &&
+              h.cols <= INT_MAX && h.depth <= INT_MAX;
+    const long long size = ok ? h.num * h.rows * h.cols * h.depth : 0;
+    ok = ok && size <= (1LL << 34);
+    float* v = ok ? static_cast<float*>(std::malloc(sizeof(float) * (size_t)size)) : nullptr;
+    ok = ok && v && std::fread(v, sizeof(float), (size_t)size, f) == (size_t)size &&
+         std::fgetc(f) == EOF;
+    std::fclose(f);
+    if (!ok) {
+        std::free(v);
+        return false;
+    }
+    dims4[0] = (int)h.rows;
+    dims4[1] = (int)h.cols;
+    dims4[2] = (int)h.depth;
+    dims4[3] = (int)h.num;
+    *data = v;
+    return true;
+}
+
+int read_text(const char* path, float** data, int dims4[4]) {
+    std::vector<char> buf;
+    int rc = slurp(path, &buf);
+    if (rc) return rc;
+    const char* p = buf.data();
+    const char* e = buf.data() + buf.size() - 1;   // before the NUL
+    // four header lines, each read with atoi semantics (cpp:256-265)
     long long hdr[4];
     for (int i = 0; i < 4; ++i) {
-        if (!std::getline(in, line)) return PCT_ERR_FORMAT;
-        hdr[i] = std::atoll(line.c_str());
+        if (p >= e) return PCT_ERR_FORMAT;
+        const char* nl = static_cast<const char*>(std::memchr(p, '\n', (size_t)(e - p)));
+        const char* line_end = nl ? nl : e;
+        hdr[i] = std::atoll(std::string(p, line_end).c_str());
+        p = nl ? nl + 1 : e;
     }
     const long long num = hdr[0], rows = hdr[1], cols = hdr[2], depth = hdr[3];
     if (num <= 0 || rows <= 0 || cols <= 0 || depth <= 0) return PCT_ERR_FORMAT;
+    if (num > INT_MAX || rows > INT_MAX || cols > INT_MAX || depth > INT_MAX)
+        return PCT_ERR_TOO_LARGE;
     const long long size = num * rows * cols * depth;
     if (size > (1LL << 34)) return PCT_ERR_TOO_LARGE;
     float* v = static_cast<float*>(std::malloc(sizeof(float) * (size_t)size));
     if (!v) return PCT_ERR_NO_MEMORY;
-    long long idx = 0;
     // values separated by blanks/newlines, in memory order (cpp:273-285)
-    std::string tok;
-    while (in >> tok) {
-        errno = 0;
-        char* end = nullptr;
-        const float f = std::strtof(tok.c_str(), &end);
-        if (end == tok.c_str() || *end != '\0') {
-            std::free(v);
-            return PCT_ERR_FORMAT;
-        }
-        if (idx >= size) {
-            std::free(v);
-            return PCT_ERR_FORMAT;
-        }
-        v[idx++] = f;
-    }
-    if (idx != size) {
+    rc = parse_values(p, e, v, size);
+    if (rc) {
         std::free(v);
-        return PCT_ERR_FORMAT;
+        return rc;
     }
     dims4[0] = (int)rows;
     dims4[1] = (int)cols;
     dims4[2] = (int)depth;
     dims4[3] = (int)num;
     *data = v;
+    return PCT_OK;
+}
+
+}  // namespace
+
+extern "C" int pct_matrixio_read(const char* path, float** data, int dims4[4]) {
+    if (!path || !data || !dims4) return PCT_ERR_INVALID;
+    *data = nullptr;
+    if (read_sidecar(path, data, dims4)) return PCT_OK;
+    return read_text(path, data, dims4);
+}
+
+extern "C" int pct_matrixio_write_sidecar(const char* path) {
+    if (!path) return PCT_ERR_INVALID;
+    long long bytes = 0, mtime = 0;
+    if (!text_stamp(path, &bytes, &mtime)) return PCT_ERR_IO;
+    float* v = nullptr;
+    int dims4[4];
+    int rc = read_text(path, &v, dims4);
+    if (rc) return rc;
+    SidecarHeader h;
+    std::memcpy(h.magic, kSidecarMagic, sizeof(kSidecarMagic));
+    h.num = dims4[3];
+    h.rows = dims4[0];
+    h.cols = dims4[1];
+    h.depth = dims4[2];
+    h.text_bytes = bytes;
+    h.text_mtime_ns = mtime;
+    const size_t size = (size_t)h.num * h.rows * h.cols * h.depth;
+    // written under a temporary name and renamed, so a reader never sees a
+    // partial sidecar
+    const std::string final_path = sidecar_path(path), tmp_path = final_path + ".tmp";
+    FILE* f = std::fopen(tmp_path.c_str(), "wb");
+    if (!f) {
+        std::free(v);
+        return PCT_ERR_IO;
+    }
+    bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 && std::fwrite(v, sizeof(float), size, f) == size;
+    ok = (std::fclose(f) == 0) && ok;
+    std::free(v);
+    if (!ok || std::rename(tmp_path.c_str(), final_path.c_str()) != 0) {
+        std::remove(tmp_path.c_str());
+        return PCT_ERR_IO;
+    }
     return PCT_OK;
 }
 
@@ -289,18 +462,29 @@ extern "C" int pct_matrixio_write(const char* path, const float* data, int num, 
     if (!f) return PCT_ERR_IO;
     const long long size = (long long)num * rows * cols * depth;
     std::fprintf(f, "%d\n%d\n%d\n%d\n", num, rows, cols, depth);
-    std::vector<char> buf(1 << 16);
-    size_t used = 0;
-    for (long long i = 0; i < size; ++i) {
-        if (buf.size() - used < 64) {
-            std::fwrite(buf.data(), 1, used, f);
-            used = 0;
+    // formatted in parallel blocks, written in order
+    constexpr long long kBlock = 1 << 16;
+    const long long nblocks = (size + kBlock - 1) / kBlock;
+    bool ok = true;
+    for (long long b0 = 0; b0 < nblocks && ok; b0 += 256) {
+        const long long b1 = std::min(nblocks, b0 + 256);
+        std::vector<std::string> text((size_t)(b1 - b0));
+#pragma omp parallel for schedule(dynamic)
+        for (long long b = b0; b < b1; ++b) {
+            std::string& t = text[(size_t)(b - b0)];
+            t.reserve((size_t)kBlock * 14);
+            char one[64];
+            const long long e = std::min(size, (b + 1) * kBlock);
+            for (long long i = b * kBlock; i < e; ++i) {
+                // std::ostream << float with the default precision 6 (cpp:307)
+                const int k = std::snprintf(one, sizeof(one), "%g ", (double)data[i]);
+                t.append(one, (size_t)k);
+            }
         }
-        // std::ostream << float with the default precision 6 (cpp:307)
-        used += (size_t)std::snprintf(buf.data() + used, 64, "%g ", (double)data[i]);
+        for (const auto& t : text)
+            if (std::fwrite(t.data(), 1, t.size(), f) != t.size()) ok = false;
     }
-    std::fwrite(buf.data(), 1, used, f);
-    const bool ok = std::ferror(f) == 0;
+    ok = ok && std::ferror(f) == 0;
     return (std::fclose(f) == 0 && ok) ? PCT_OK : PCT_ERR_IO;
 }
 

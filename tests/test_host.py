@@ -43,6 +43,61 @@ def test_matrixio_read_matches_reference(ref, tmp_path):
     np.testing.assert_allclose(ours, a, rtol=1e-5)
 
 
+def test_matrixio_read_at_scale_matches_reference(ref, tmp_path):
+    """Two million values over many lines, mixed blanks, CRLF and '+' signs:
+    the parallel parser cuts this text into many chunks."""
+    rng = np.random.default_rng(2)
+    a = (rng.standard_normal(2_000_000) * 300).astype(np.float32)
+    toks = [f"{v:.9g}" for v in a[:1000]] + [f"{v:g}" for v in a[1000:]]
+    pos = int(np.nonzero(a[:1000] > 0)[0][0])
+    toks[pos] = "+" + toks[pos]
+    seps = rng.choice([" ", "  ", "\t", "\n", "\r\n", " \n"], size=len(toks))
+    p = str(tmp_path / "big.txt")
+    with open(p, "w", newline="") as f:
+        f.write("2\n100\n100\n100\n")
+        f.write("".join(t + s for t, s in zip(toks, seps)))
+    ours, dims = pct.matrixio_read(p)
+    theirs, rdims = oracle.ref_matrixio_read(p)
+    assert dims == rdims == [100, 100, 100, 2]
+    assert np.array_equal(bits(ours), bits(theirs))
+
+
+def test_matrixio_sidecar(tmp_path):
+    rng = np.random.default_rng(3)
+    a = (rng.random(4 * 5 * 6 * 2) * 255).astype(np.float32)
+    p = str(tmp_path / "m.txt")
+    pct.matrixio_write(p, a, 2, 4, 5, 6)
+    text, dims = pct.matrixio_read(p)
+    pct.matrixio_write_sidecar(p)
+    side = p + ".pctbin"
+    assert os.path.exists(side) and not os.path.exists(side + ".tmp")
+    got, gdims = pct.matrixio_read(p)
+    assert gdims == dims and np.array_equal(bits(got), bits(text))
+    # the sidecar's values are what is read while the text is unchanged
+    raw = bytearray(open(side, "rb").read())
+    payload = np.frombuffer(bytes(raw[-4 * a.size:]), np.float32).copy()
+    payload[7] = 12345.0
+    raw[-4 * a.size:] = payload.tobytes()
+    open(side, "wb").write(bytes(raw))
+    got, _ = pct.matrixio_read(p)
+    assert got[7] == 12345.0 and np.array_equal(np.delete(got, 7), np.delete(text, 7))
+    # a changed text file makes the sidecar stale
+    b = a * 0.5
+    pct.matrixio_write(p, b, 2, 4, 5, 6)
+    st = os.stat(p)
+    os.utime(p, ns=(st.st_atime_ns, st.st_mtime_ns + 1_000_000_000))
+    got, _ = pct.matrixio_read(p)
+    np.testing.assert_allclose(got, b, rtol=1e-5)
+    # a truncated sidecar is ignored
+    pct.matrixio_write_sidecar(p)
+    open(side, "r+b").truncate(100)
+    got2, _ = pct.matrixio_read(p)
+    assert np.array_equal(bits(got2), bits(got))
+    with pytest.raises(pct.PctError) as e:
+        pct.matrixio_write_sidecar(str(tmp_path / "missing.txt"))
+    assert e.value.status == _lib.PCT_ERR_IO
+
+
 def test_matrixio_errors(tmp_path):
     p = str(tmp_path / "bad.txt")
     with open(p, "w") as f:
