@@ -353,16 +353,26 @@ def main() -> int:
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
         "traffic": None,
-        "kernel": "fuse_tiled",
+        "kernel": None,
         "kernel_ms": round(kernel_ms, 4),
         "flops_per_launch": flops,
         "cvp_per_launch": cvp,
         "algorithmic_bytes": 4 * full * (2 + 2 * n),
     }
     tfile = os.path.join(ROOT, "profiles", "fuse_tiled_222x222x112_n10_nh11_traffic.json")
+    timed_kernel = (f"fuse_tiled<{cfg['seg_width']},{cfg['bx']},{cfg['bz']},{cfg['waves']},"
+                    f"{(nh - 1) // 2}>")
+    roofline["kernel"] = timed_kernel
+    prof = None
     if (nz, nx, ny, n, nh) == (112, 222, 222, 10, 11) and os.path.exists(tfile):
         with open(tfile) as f:
             prof = json.load(f)
+        if prof.get("kernel") != timed_kernel:
+            # counters of another tile shape do not describe this launch
+            roofline["traffic_note"] = (f"profiled kernel {prof.get('kernel')} is not the timed "
+                                        f"kernel {timed_kernel}")
+            prof = None
+    if prof is not None:
         roofline["traffic"] = prof.get("hbm_bytes_per_launch")
         if "valu" in prof:
             v = prof["valu"]

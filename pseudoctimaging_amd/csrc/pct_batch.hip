@@ -343,28 +343,35 @@ static pct_batch_choice model_choice(const pct_subject* s, int n) {
     pct_batch_choice ch = empty_choice();
     ch.serial = 1;
     for (int hw = 1; hw <= kMaxTiledHalfWindow; ++hw) {
+        // the measured preference of the window where it has a persistent
+        // kernel, else every batched shape
+        const int pref = preferred_shape(hw);
         double best = -1;
-        for (const auto& bk : batch_kernels())
-            for (int grp : class_groups(s, n, hw)) {
-                double cost = 0;
-                bool any = false, ok = true;
-                for (int i = 0; i < n && ok; ++i) {
-                    if (!has_items(s[i].pb) || class_of(s[i].pb) != hw) continue;
-                    TileGeometry g;
-                    const double t = model_item_time(&s[i].pb, bk.shape_id,
-                                                     std::min(grp, s[i].pb.n_atlas), &g);
-                    if (t < 0) {
-                        ok = false;
-                        break;
+        for (int pass = (pref >= 0 && find_batch_kernel(pref)) ? 0 : 1; pass < 2 && best < 0;
+             ++pass)
+            for (const auto& bk : batch_kernels()) {
+                if (pass == 0 && bk.shape_id != pref) continue;
+                for (int grp : class_groups(s, n, hw)) {
+                    double cost = 0;
+                    bool any = false, ok = true;
+                    for (int i = 0; i < n && ok; ++i) {
+                        if (!has_items(s[i].pb) || class_of(s[i].pb) != hw) continue;
+                        TileGeometry g;
+                        const double t = model_item_time(&s[i].pb, bk.shape_id,
+                                                         std::min(grp, s[i].pb.n_atlas), &g);
+                        if (t < 0) {
+                            ok = false;
+                            break;
+                        }
+                        cost += t * (double)g.n_items / ((double)kNumCu * g.wg_per_cu);
+                        any = true;
                     }
-                    cost += t * (double)g.n_items / ((double)kNumCu * g.wg_per_cu);
-                    any = true;
-                }
-                if (!ok || !any) continue;
-                if (best < 0 || cost < best * 0.999) {
-                    best = cost;
-                    ch.shape_id[hw] = bk.shape_id;
-                    ch.atlas_group[hw] = grp;
+                    if (!ok || !any) continue;
+                    if (best < 0 || cost < best * 0.999) {
+                        best = cost;
+                        ch.shape_id[hw] = bk.shape_id;
+                        ch.atlas_group[hw] = grp;
+                    }
                 }
             }
     }

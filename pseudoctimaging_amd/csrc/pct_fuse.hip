@@ -294,6 +294,20 @@ static double model_cost(const pct_problem* pb, int shape_id, int group, TileGeo
     return rounds * item_time;
 }
 
+// Tile shape that measured fastest on MI355X for a half window, or -1 where
+// the cost model decides.  Autotuning at 222x222x112 (n=10, NH=11; n=5 and
+// 7, NH=11), 192x192x96 and 256x256x128 (NH=9), 160x160x96 and 128x128x64
+// (NH=7): the analytic model is within about 15 % of the measured times,
+// while the best shapes of a window lie within 5 % of each other.
+int preferred_shape(int hw) {
+    switch (hw) {
+        case 3: return 11;   // {32, 2, 2, 4}: 0.42 ms vs 0.45 next (160x160x96, n=1)
+        case 4: return 9;    // {64, 2, 4, 4}: 21.2 ms vs 21.7 next (256x256x128, n=10)
+        case 5: return 7;    // {32, 2, 4, 4}: 22.3 ms vs 23.3 next (222x222x112, n=10)
+        default: return -1;
+    }
+}
+
 struct ShapeKey {
     int nx, ny, nz, n, nh;
     bool operator<(const ShapeKey& o) const {
@@ -325,17 +339,24 @@ static bool choose(const pct_problem* pb, int* shape_id, int* group, double* cos
             return c >= 0;
         }
     }
+    // a measured preference for the window fixes the shape and the model
+    // picks the atlas group; without one (or where it does not fit) the model
+    // picks both
+    const int pref = preferred_shape((pb->nh - 1) / 2);
     double best = -1;
-    for (int sid = 0; sid < (int)shapes().size(); ++sid)
-        for (int grp = 1; grp <= pb->n_atlas; ++grp) {
-            TileGeometry g;
-            const double c = model_cost(pb, sid, grp, &g);
-            if (c < 0) continue;
-            if (best < 0 || c < best * 0.999) {
-                best = c;
-                *shape_id = sid;
-                *group = grp;
-                if (geo) *geo = g;
+    for (int pass = pref >= 0 ? 0 : 1; pass < 2 && best < 0; ++pass)
+        for (int sid = 0; sid < (int)shapes().size(); ++sid) {
+            if (pass == 0 && sid != pref) continue;
+            for (int grp = 1; grp <= pb->n_atlas; ++grp) {
+                TileGeometry g;
+                const double c = model_cost(pb, sid, grp, &g);
+                if (c < 0) continue;
+                if (best < 0 || c < best * 0.999) {
+                    best = c;
+                    *shape_id = sid;
+                    *group = grp;
+                    if (geo) *geo = g;
+                }
             }
         }
     if (cost) *cost = best;

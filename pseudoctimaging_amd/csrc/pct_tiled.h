@@ -418,6 +418,9 @@ bool make_geometry(const pct_problem* pb, int shape_id, int group, TileGeometry*
 // for every shape), or -1 where the shape has no geometry for the problem.
 double model_item_time(const pct_problem* pb, int shape_id, int group, TileGeometry* geo);
 
+// Tile shape that measured fastest for a half window, or -1.
+int preferred_shape(int hw);
+
 constexpr int kNumCu = 256;
 
 // Checks that the current device is a gfx950; returns its ordinal.
