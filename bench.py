@@ -144,7 +144,7 @@ def batched_leg(steps: int, warmup: int, seed: int):
 
 def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup, seed,
                 plain_ms, train_shape=(64, 96, 96), train_subjects=4, train_rounds=2,
-                train_steps=60, topk=32, samples=27, margin=0.0, max_slabs=4):
+                train_steps=60, topk=32, samples=27, margin=0.0, max_slabs=4, shape_id=-1):
     """Learned-pruned fusion on the resident inputs: the embedding MLP is
     trained on the GPU on `train_subjects` other seeded phantoms, then the
     pruned fusion is timed and compared with the plain output."""
@@ -173,7 +173,7 @@ def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup,
     train_s = time.perf_counter() - t0
     del subjects
     out = torch.empty_like(plain_out)
-    kw = dict(topk=topk, samples=samples, margin=margin, max_slabs=max_slabs)
+    kw = dict(topk=topk, samples=samples, margin=margin, max_slabs=max_slabs, shape_id=shape_id)
     frac = learned.fuse_pruned_device(d_img, d_amri, d_lab, out, nh, sigma, beta, params, **kw)
     for _ in range(warmup):
         learned.fuse_pruned_device(d_img, d_amri, d_lab, out, nh, sigma, beta, params,

@@ -13,9 +13,11 @@
 //  box_pass       NH-wide box sums of the atlas CT (three separable passes).
 //  select_kernel  per (tile, atlas): embedding distance of every window
 //                 offset summed over a 4x2x2 probe cluster at the tile centre
-//                 (atlas embeddings of the cluster's window staged in LDS),
-//                 bitonic sort, the topk offsets kept, a low score gap
-//                 flags the item for the full search.
+//                 (probe-by-voxel dot products on BF16 MFMA, one window
+//                 plane at a time), the topk offsets kept (radix select; a
+//                 bitonic sort where a positive margin needs the order
+//                 statistics), a low score gap flags the item for the full
+//                 search.
 //  pruned_kernel  per (tile, atlas): the kept offsets and a 3x3x3 lattice of
 //                 sample offsets, weighted exactly with the same LDS slabs and
 //                 separable box sums as the plain tiled kernel; the other
@@ -665,14 +667,150 @@ struct SelectArgs {
 constexpr int kSelThreads = 512;
 constexpr int kSelPerThread = 8;    // 4096 offsets (NH = 15: 3375)
 constexpr int kNoOffset = 1 << 30;
-constexpr int kSelBatch = 8;        // window loads in flight per thread
 
 // One workgroup per item = (atlas, tile), item = atlas * n_tiles + tile.
+// Order-preserving integer image of a float score (-0 is taken as +0, as the
+// float comparison does).
+__device__ __forceinline__ unsigned order_key(float f) {
+    const unsigned u = __float_as_uint(f + 0.f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// The topk smallest (score, offset index) pairs among the m candidates of the
+// allowed slabs, written in ascending offset order, and the item's flag for a
+// margin <= 0 (set for a negative margin).  Thread t holds the scores of
+// offsets t + q kSelThreads.  A four-pass radix select over the order keys
+// finds T, the key of the k-th smallest; keys below T are kept, and of the
+// keys equal to T the ones with the smallest offsets.  Ordered compaction by
+// wave ballots writes the list.  The set equals the first k entries of the
+// full sort by (score, offset index).
+__device__ __forceinline__ void select_topk_radix(
+    const SelectArgs& a, int item, int ncand, int nw2, const float (&sc)[kSelPerThread],
+    const float* key, unsigned allowed_bits, int m, unsigned* hist, unsigned& sh_T,
+    int& sh_need, int& sh_cnt, int* tie, int* wcnt) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = min(a.topk, m);
+    bool al[kSelPerThread];
+    unsigned uk[kSelPerThread];
+#pragma unroll
+    for (int q = 0; q < kSelPerThread; ++q) {
+        const int c = threadIdx.x + q * kSelThreads;
+        al[q] = c < ncand && ((allowed_bits >> (c / nw2)) & 1u);
+        uk[q] = order_key(sc[q]);
+    }
+    unsigned T = 0;
+    int need = k, n_eq = k;
+    if (k < m) {
+        for (int shift = 24; shift >= 0; shift -= 8) {
+            if (threadIdx.x < 256) hist[threadIdx.x] = 0;
+            __syncthreads();
+            const unsigned hi_mask = shift == 24 ? 0u : (0xffffffffu << (shift + 8));
+#pragma unroll
+            for (int q = 0; q < kSelPerThread; ++q)
+                if (al[q] && ((uk[q] ^ T) & hi_mask) == 0)
+                    atomicAdd(&hist[(uk[q] >> shift) & 255u], 1u);
+            __syncthreads();
+            if (wv == 0) {
+                // lane l holds bins 4l .. 4l+3; the bin where the running
+                // count reaches `need` extends T by one byte
+                const unsigned h0 = hist[4 * lane], h1 = hist[4 * lane + 1],
+                               h2 = hist[4 * lane + 2], h3 = hist[4 * lane + 3];
+                const unsigned own = h0 + h1 + h2 + h3;
+                unsigned incl = own;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const unsigned v = __shfl_up(incl, o);
+                    if (lane >= o) incl += v;
+                }
+                const unsigned excl = incl - own;
+                const unsigned nd = (unsigned)need;
+                if (excl < nd && nd <= incl) {
+                    unsigned before = excl, cnt = h0;
+                    int j = 0;
+                    if (before + h0 < nd) {
+                        before += h0;
+                        j = 1;
+                        cnt = h1;
+                        if (before + h1 < nd) {
+                            before += h1;
+                            j = 2;
+                            cnt = h2;
+                            if (before + h2 < nd) {
+                                before += h2;
+                                j = 3;
+                                cnt = h3;
+                            }
+                        }
+                    }
+                    sh_T = T | ((unsigned)(4 * lane + j) << shift);
+                    sh_need = (int)(nd - before);
+                    sh_cnt = (int)cnt;
+                }
+            }
+            __syncthreads();
+            T = sh_T;
+            need = sh_need;
+            n_eq = sh_cnt;
+        }
+        if (need < n_eq) {
+            // more keys equal T than places: the smallest offsets win
+            if (threadIdx.x == 0) {
+                int taken = 0;
+                for (int c = 0; c < ncand && taken < need; ++c)
+                    if (((allowed_bits >> (c / nw2)) & 1u) && order_key(key[c]) == T)
+                        tie[taken++] = c;
+            }
+            __syncthreads();
+        }
+    }
+    bool keep[kSelPerThread];
+#pragma unroll
+    for (int q = 0; q < kSelPerThread; ++q) {
+        const int c = threadIdx.x + q * kSelThreads;
+        bool kp = al[q] && (k == m || uk[q] < T);
+        if (al[q] && k < m && uk[q] == T) {
+            if (need == n_eq) {
+                kp = true;
+            } else {
+                for (int j = 0; j < need; ++j) kp |= tie[j] == c;
+            }
+        }
+        keep[q] = kp;
+    }
+    int* out = a.sel + (long long)item * a.topk;
+    int base = 0;
+#pragma unroll
+    for (int q = 0; q < kSelPerThread; ++q) {
+        if (q * kSelThreads >= ncand) break;   // uniform over the workgroup
+        const unsigned long long bal = __ballot(keep[q]);
+        if (lane == 0) wcnt[wv] = __popcll(bal);
+        __syncthreads();
+        int off = base, tot = 0;
+#pragma unroll
+        for (int w = 0; w < kSelThreads / 64; ++w) {
+            if (w < wv) off += wcnt[w];
+            tot += wcnt[w];
+        }
+        if (keep[q])
+            out[off + __popcll(bal & ((1ull << lane) - 1ull))] = threadIdx.x + q * kSelThreads;
+        __syncthreads();   // wcnt is rewritten for the next q
+        base += tot;
+    }
+    for (int i = k + threadIdx.x; i < a.topk; i += kSelThreads) out[i] = kNoOffset;
+    if (threadIdx.x == 0) a.flag[item] = (unsigned char)(a.margin < 0.f ? 1 : 0);
+}
+
 __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
     extern __shared__ float sh[];
     __shared__ float dzmin[16];
     __shared__ int allowed[16];       // allowed dz indices, ascending
     __shared__ int n_allowed;
+    __shared__ unsigned allowed_bits;
+    __shared__ unsigned rs_hist[256];      // radix-select histogram
+    __shared__ unsigned rs_T;
+    __shared__ int rs_need, rs_cnt;
+    __shared__ int rs_tie[kMaxTopk];
+    __shared__ int rs_wcnt[kSelThreads / 64];
     const int item = blockIdx.x;
     const int at = item / a.n_tiles;
     int t = item - at * a.n_tiles;
@@ -683,97 +821,112 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
     const int r = a.r, hw = a.hw, nw = 2 * hw + 1, nw2 = nw * nw;
     const int ncand = nw2 * nw;
     // probe cluster: 4 (y) x 2 (x) x 2 (z) voxels at the centre of the
-    // tile's part of the interior
+    // tile's part of the interior; probe p sits at (p & 3, (p >> 2) & 1,
+    // p >> 3).  Its window spans WY x WXX x WZ atlas voxels.  Coordinates are
+    // clamped into the volume (a thin interior can leave the window partly
+    // outside).
     const int py0 = probe_origin(ty, a.oy, r, a.ny, 4);
     const int px0 = probe_origin(tx, a.wx, r, a.nx, 2);
     const int pz0 = probe_origin(tz, a.bz, r, a.nz, 2);
-    // LDS: probe target embeddings (16 x 16 floats), then the atlas
-    // embeddings of the cluster's window as 8 planes of BF16 pairs (plane k2
-    // holds dimensions 2 k2, 2 k2 + 1 of every window voxel: lanes scoring
-    // neighbouring offsets read neighbouring words).  Coordinates are clamped
-    // into the volume (a thin interior can leave the window partly outside).
-    unsigned* sU = reinterpret_cast<unsigned*>(sh);          // [16 probes][8] BF16 pairs of -2 t
-    unsigned* sA = reinterpret_cast<unsigned*>(sh + 256);
     const int WY = 4 + 2 * hw, WXX = 2 + 2 * hw, WZ = 2 + 2 * hw;
-    const int nwin = WY * WXX * WZ;
+    const int pv = WY * WXX;                   // voxels per window plane
+    const int ntile = (pv + 15) >> 4;          // 16-voxel MFMA tiles per plane
+    const int pvs = ntile << 4;                // plane stride of the G ring
     const long long plane = (long long)a.nx * a.ny;
-    for (int i = threadIdx.x; i < 128; i += kSelThreads) {
-        const int p = i >> 3, k2 = i & 7;
-        const int yy = min(py0 + (p & 3), a.ny - 1), xx = min(px0 + ((p >> 2) & 1), a.nx - 1),
-                  zz = min(pz0 + (p >> 3), a.nz - 1);
-        const unsigned short* e = a.embT + (zz * plane + (long long)xx * a.ny + yy) * 16 + 2 * k2;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ln = lane & 15, lg = lane >> 4;
+    // The score of offset d is sum_p |t_p - a(p + d)|^2 less the common
+    // sum_p |t_p|^2, i.e. sum_p G[p][p + d] with G[p][w] = -2 t_p . a_w +
+    // |a_w|^2.  Per window plane, one BF16 MFMA 16x16x16 gives -2 t_p . a_w
+    // for the 16 probes and 16 voxels; |a_w|^2 is added in FP32.  G of two
+    // consecutive planes lives in an LDS ring; the offsets of one dz read the
+    // plane of the z = 0 probes and the next one.
+    float* key = sh;                                   // [ncand] scores by offset
+    float* ring = sh + ((ncand + 63) & ~63);           // [2][16][pvs]
+    v4s aop;   // A operand: lane holds -2 t[p = ln][k = 4 lg .. 4 lg + 3]
+    {
+        const int yy = min(py0 + (ln & 3), a.ny - 1), xx = min(px0 + ((ln >> 2) & 1), a.nx - 1),
+                  zz = min(pz0 + (ln >> 3), a.nz - 1);
+        const uint2 raw = *reinterpret_cast<const uint2*>(
+            a.embT + (zz * plane + (long long)xx * a.ny + yy) * 16 + 4 * lg);
+        const v4s t = __builtin_bit_cast(v4s, raw);
         // -2 t is exact in BF16
-        const unsigned lo = f2bf(-2.f * bf2f(e[0])), hi = f2bf(-2.f * bf2f(e[1]));
-        sU[i] = lo | (hi << 16);
+        aop = (v4s){(short)f2bf(-2.f * bf2f((unsigned short)t[0])),
+                    (short)f2bf(-2.f * bf2f((unsigned short)t[1])),
+                    (short)f2bf(-2.f * bf2f((unsigned short)t[2])),
+                    (short)f2bf(-2.f * bf2f((unsigned short)t[3]))};
     }
-    // two 16-byte loads per window voxel, kSelBatch of them in flight per
-    // thread before any LDS store
-    const uint4* __restrict__ eA =
-        reinterpret_cast<const uint4*>(a.embA + (long long)at * a.N * 16);
-    for (int i0 = threadIdx.x; i0 < nwin * 2; i0 += kSelThreads * kSelBatch) {
-        uint4 v[kSelBatch];
+    const unsigned short* __restrict__ eA = a.embA + (long long)at * a.N * 16;
+    // B operands of one plane: wave wv takes tiles wv, wv + 8, ...; lane
+    // holds dims 4 lg .. 4 lg + 3 of voxel 16 tile + ln
+    constexpr int kTilesPerWave = 3;   // 18 tiles at NH = 15
+    auto load_plane = [&](int wz, uint2 (&b)[kTilesPerWave]) {
+        const int zz = min(max(pz0 - hw + wz, 0), a.nz - 1);
 #pragma unroll
-        for (int b = 0; b < kSelBatch; ++b) {
-            const int i = min(i0 + b * kSelThreads, nwin * 2 - 1);
-            const int w = i >> 1, h = i & 1;
-            const int wy = w % WY, wx = (w / WY) % WXX, wz = w / (WY * WXX);
-            const int yy = min(max(py0 - hw + wy, 0), a.ny - 1);
-            const int xx = min(max(px0 - hw + wx, 0), a.nx - 1);
-            const int zz = min(max(pz0 - hw + wz, 0), a.nz - 1);
-            v[b] = eA[(zz * plane + (long long)xx * a.ny + yy) * 2 + h];
-        }
-#pragma unroll
-        for (int b = 0; b < kSelBatch; ++b) {
-            const int i = i0 + b * kSelThreads;
-            if (i < nwin * 2) {
-                const int w = i >> 1, k0 = (i & 1) * 4;
-                sA[(k0 + 0) * nwin + w] = v[b].x;
-                sA[(k0 + 1) * nwin + w] = v[b].y;
-                sA[(k0 + 2) * nwin + w] = v[b].z;
-                sA[(k0 + 3) * nwin + w] = v[b].w;
+        for (int i = 0; i < kTilesPerWave; ++i) {
+            const int tile = wv + i * (kSelThreads / 64);
+            const int wl = tile * 16 + ln;
+            uint2 v = make_uint2(0u, 0u);
+            if (tile < ntile && wl < pv) {
+                const int wy = wl % WY, wx = wl / WY;
+                const int yy = min(max(py0 - hw + wy, 0), a.ny - 1);
+                const int xx = min(max(px0 - hw + wx, 0), a.nx - 1);
+                v = *reinterpret_cast<const uint2*>(
+                    eA + (zz * plane + (long long)xx * a.ny + yy) * 16 + 4 * lg);
             }
+            b[i] = v;
         }
+    };
+    auto plane_g = [&](int slot, const uint2 (&b)[kTilesPerWave]) {
+        float* g = ring + slot * 16 * pvs;
+#pragma unroll
+        for (int i = 0; i < kTilesPerWave; ++i) {
+            const int tile = wv + i * (kSelThreads / 64);
+            if (tile >= ntile) break;   // uniform over the wave
+            const v4s bop = __builtin_bit_cast(v4s, b[i]);
+            const v4f d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                aop, bop, (v4f){0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            float nn = 0.f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float v = bf2f((unsigned short)bop[j]);
+                nn = fmaf(v, v, nn);
+            }
+            nn += __shfl_xor(nn, 16);
+            nn += __shfl_xor(nn, 32);
+            const int wl = tile * 16 + ln;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) g[(4 * lg + j) * pvs + wl] = d[j] + nn;
+        }
+    };
+    uint2 bq[kTilesPerWave];
+    load_plane(0, bq);
+    plane_g(0, bq);
+    load_plane(1, bq);
+    for (int dzi = 0; dzi < nw; ++dzi) {
+        plane_g((dzi + 1) & 1, bq);
+        if (dzi + 2 < WZ) load_plane(dzi + 2, bq);   // in flight during the scores
+        __syncthreads();
+        if ((int)threadIdx.x < nw2) {
+            const int dxi = threadIdx.x / nw, dyi = threadIdx.x - dxi * nw;
+            const float* g0 = ring + (dzi & 1) * 16 * pvs;         // probes with z = 0
+            const float* g1 = ring + ((dzi + 1) & 1) * 16 * pvs;   // probes with z = 1
+            float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int p = 0; p < 16; ++p) {
+                const float* g = (p >> 3) ? g1 : g0;
+                acc[p & 3] += g[p * pvs + (((p >> 2) & 1) + dxi) * WY + (p & 3) + dyi];
+            }
+            key[dzi * nw2 + threadIdx.x] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+        }
+        __syncthreads();   // the ring slot is rewritten next
     }
-    __syncthreads();
-    // scores: thread handles offsets c = threadIdx.x + q * kSelThreads; the
-    // probe loop stays rolled so the unrolled q loop keeps a small code body
     float sc[kSelPerThread];
 #pragma unroll
     for (int q = 0; q < kSelPerThread; ++q) {
         const int c = threadIdx.x + q * kSelThreads;
-        float score = __builtin_huge_valf();
-        if (c < ncand) {
-            // sum over the probes of |a|^2 - 2 t.a, which orders the offsets
-            // like the sum of |t - a|^2 (the |t|^2 terms are common to every
-            // offset); two BF16 dot products per dimension pair
-            const int dz = c / nw2 - hw, dx = (c / nw) % nw - hw, dy = c % nw - hw;
-            float acc[4] = {0.f, 0.f, 0.f, 0.f};   // independent chains
-#pragma unroll 1
-            for (int p = 0; p < 16; ++p) {
-                const int wy = (p & 3) + hw + dy, wx = ((p >> 2) & 1) + hw + dx,
-                          wz = (p >> 3) + hw + dz;
-                const unsigned* ea = sA + (wz * WXX + wx) * WY + wy;
-                const unsigned* eu = sU + p * 8;
-#pragma unroll
-                for (int k2 = 0; k2 < 8; ++k2) {
-                    const bf2 av = __builtin_bit_cast(bf2, ea[k2 * nwin]);
-                    const bf2 uv = __builtin_bit_cast(bf2, eu[k2]);
-                    acc[k2 & 3] = __builtin_amdgcn_fdot2_f32_bf16(av, av, acc[k2 & 3], false);
-                    acc[k2 & 3] = __builtin_amdgcn_fdot2_f32_bf16(av, uv, acc[k2 & 3], false);
-                }
-            }
-            score = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-        }
-        sc[q] = score;
+        sc[q] = c < ncand ? key[c] : __builtin_huge_valf();
     }
-    __syncthreads();   // the window is dead from here on
-    float* key = sh;   // [ncand] scores by offset index
-#pragma unroll
-    for (int q = 0; q < kSelPerThread; ++q) {
-        const int c = threadIdx.x + q * kSelThreads;
-        if (c < ncand) key[c] = sc[q];
-    }
-    __syncthreads();
     // best score of every dz slab (16 threads per slab, then a shuffle
     // reduction), then the allowed slabs: the lattice's and the best others
     // up to max_slabs (ties to the smaller dz)
@@ -807,11 +960,24 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
             ++cnt;
         }
         int m = 0;
+        unsigned bits = 0;
         for (int d = 0; d < nw; ++d)
-            if (ok[d]) allowed[m++] = d;
+            if (ok[d]) {
+                allowed[m++] = d;
+                bits |= 1u << d;
+            }
         n_allowed = m;
+        allowed_bits = bits;
     }
     __syncthreads();
+    if (a.margin <= 0.f) {
+        // The flag does not depend on the order statistics here, so the kept
+        // set - the topk smallest (score, offset index) pairs of the allowed
+        // slabs - comes from a radix select, without a sort.
+        select_topk_radix(a, item, ncand, nw2, sc, key, allowed_bits, n_allowed * nw2, rs_hist,
+                          rs_T, rs_need, rs_cnt, rs_tie, rs_wcnt);
+        return;
+    }
     // candidates of the allowed slabs, then bitonic sort ascending by
     // (score, offset index)
     const int m = n_allowed * nw2;
@@ -955,7 +1121,10 @@ template <int W, int BX, int BZ, int NW, int HW>
 __global__ __launch_bounds__(NW * 64) void pruned_kernel(PrunedArgs pa) {
     using C = TiledCfg<W, BX, BZ, NW, HW>;
     extern __shared__ float lds[];
-    __shared__ int offs[kMaxOffsets];     // merged offset list, ascending
+    // merged offset list: per dz plane (ascending), the kept offsets, then
+    // the lattice samples that are not kept; each entry's dz and LDS delta
+    __shared__ int odz[kMaxOffsets];
+    __shared__ int odel[kMaxOffsets];
     __shared__ int is_sample[kMaxOffsets];
     __shared__ int kept[kMaxTopk], smp[kSamples];
     __shared__ int n_offs, n_samp_eff;
@@ -974,19 +1143,29 @@ __global__ __launch_bounds__(NW * 64) void pruned_kernel(PrunedArgs pa) {
     if (threadIdx.x < pa.n_samples) smp[threadIdx.x] = pa.samples[threadIdx.x];
     __syncthreads();
     if (threadIdx.x == 0) {
-        // merge the kept offsets with the lattice samples (both ascending);
-        // a sample that is also kept counts once, as kept
+        // both lists are ascending (dz-major).  Per dz plane: its kept
+        // offsets, then its samples; a sample that is also kept counts once,
+        // as kept.  Every step of an inner loop below then runs the same code.
+        constexpr int NW1 = 2 * HW + 1, NW2 = NW1 * NW1;
         int i = 0, j = 0, m = 0, ns = 0;
-        for (;;) {
-            const int a = i < pa.topk ? kept[i] : kNoOffset;
-            const int b = j < pa.n_samples ? smp[j] : kNoOffset;
-            if (a >= kNoOffset && b >= kNoOffset) break;
-            if (a < b) {
-                offs[m] = a; is_sample[m] = 0; ++m; ++i;
-            } else if (b < a) {
-                offs[m] = b; is_sample[m] = 1; ++m; ++j; ++ns;
-            } else {
-                offs[m] = a; is_sample[m] = 0; ++m; ++i; ++j;
+        auto push = [&](int o, int smpl) {
+            odz[m] = o / NW2 - HW;
+            odel[m] = ((o / NW1) % NW1) * C::CW + o % NW1;
+            is_sample[m] = smpl;
+            ++m;
+        };
+        for (int d = 0; d < NW1; ++d) {
+            const int hi = (d + 1) * NW2;
+            const int i0 = i;
+            while (i < pa.topk && kept[i] < hi) push(kept[i++], 0);
+            int t = i0;
+            while (j < pa.n_samples && smp[j] < hi) {
+                while (t < i && kept[t] < smp[j]) ++t;
+                if (!(t < i && kept[t] == smp[j])) {
+                    push(smp[j], 1);
+                    ++ns;
+                }
+                ++j;
             }
         }
         n_offs = m;
@@ -1019,7 +1198,6 @@ __global__ __launch_bounds__(NW * 64) void pruned_kernel(PrunedArgs pa) {
     const int lane_base = sg * BX * C::CW + ly;
     const int xs = x0 - 1 - HW;
     const int ys = y0 - 1 - HW;
-    constexpr int NWIN = 2 * HW + 1;
     DirectPlan<C, NW> dp;
     dp.init(xs, ys, nx, ny, wv, lane);
     const float* __restrict__ As = pa.atlas_scaled + (long long)at * args.N;
@@ -1027,17 +1205,13 @@ __global__ __launch_bounds__(NW * 64) void pruned_kernel(PrunedArgs pa) {
     const int m = n_offs;
     int k = 0;
     while (k < m) {
-        const int dz = offs[k] / (NWIN * NWIN) - HW;
+        const int dz = odz[k];
         __syncthreads();
         stage_slab_direct<C, NW>(lds, As, L, z0 + dz, nz, plane, wv, dp);
-        for (; k < m && offs[k] / (NWIN * NWIN) - HW == dz; ++k) {
-            const int dx = (offs[k] / NWIN) % NWIN, dy = offs[k] % NWIN;   // 0 .. 2HW
-            const int base = lane_base + dx * C::CW + dy;
-            if (is_sample[k])
-                pruned_step<C, BX, BZ, true>(lds, base, tv, P, S, Le, Ws);
-            else
-                pruned_step<C, BX, BZ, false>(lds, base, tv, P, S, Le, Ws);
-        }
+        for (; k < m && odz[k] == dz && !is_sample[k]; ++k)
+            pruned_step<C, BX, BZ, false>(lds, lane_base + odel[k], tv, P, S, Le, Ws);
+        for (; k < m && odz[k] == dz; ++k)
+            pruned_step<C, BX, BZ, true>(lds, lane_base + odel[k], tv, P, S, Le, Ws);
     }
 
     // tail: the candidates not evaluated, at the mean sample weight
@@ -1262,10 +1436,11 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
                           : std::min(nw, std::max(opts->max_slabs, sa.n_lattice_dz));
     int p2 = 1;
     while (p2 < slabs * nw * nw) p2 <<= 1;
-    const int win = (4 + 2 * hw) * (2 + 2 * hw) * (2 + 2 * hw);
-    // phase 1: probe embeddings and window; phase 2: scores and sort arrays
-    const size_t sel_lds = std::max(256 * 4 + (size_t)win * 32,
-                                    (size_t)((ncand + 63) & ~63) * 4 + (size_t)p2 * 8);
+    // scores by offset, then either the G ring of two window planes (while
+    // scoring) or the sort arrays (after it)
+    const int ring_stride = (((4 + 2 * hw) * (2 + 2 * hw) + 15) / 16) * 16;
+    const size_t sel_lds = (size_t)((ncand + 63) & ~63) * 4 +
+                           std::max((size_t)2 * 16 * ring_stride * 4, (size_t)p2 * 8);
     if (sel_lds > (size_t)kLdsPerCu) return PCT_ERR_UNSUPPORTED;
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(select_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sel_lds);

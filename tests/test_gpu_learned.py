@@ -165,6 +165,30 @@ def test_pruned_full_fallback_is_bit_exact(gpu):
     assert np.array_equal(out.cpu().numpy().view(np.uint32), ref.cpu().numpy().view(np.uint32))
 
 
+@pytest.mark.parametrize("case", [((30, 40, 44), 2, 11, 32, 4), ((26, 36, 30), 3, 9, 16, 0),
+                                  ((24, 30, 28), 2, 7, 64, 2), ((20, 22, 26), 1, 5, 7, 4)])
+def test_radix_selection_equals_sorted_selection(gpu, case):
+    # margin 0 selects the kept offsets by radix select; a positive margin
+    # needs the order statistics and sorts.  Both keep the topk smallest
+    # (score, offset) pairs, so where the sorting path flags no item the
+    # outputs are bit-identical.
+    import torch
+    shape, n, nh, topk, slabs = case
+    print(f"case radix vs sort {case}", flush=True)
+    img, amri, lab = _fusion_case(shape, n, nh, 24)
+    p = _random_params(5, *learned.intensity_stats(img))
+    dimg, dam, dlab = _dev(img), _dev(amri), _dev(lab)
+    fast = torch.full(img.shape, -1.0, device="cuda:0")
+    sort = torch.full(img.shape, -2.0, device="cuda:0")
+    kw = dict(topk=topk, samples=27, shape_id=13, max_slabs=slabs)
+    f0 = learned.fuse_pruned_device(dimg, dam, dlab, fast, nh, 2.5, 1.0, p, margin=0.0, **kw)
+    f1 = learned.fuse_pruned_device(dimg, dam, dlab, sort, nh, 2.5, 1.0, p, margin=1e-37, **kw)
+    torch.cuda.synchronize()
+    assert f0 == 0.0 and f1 == 0.0
+    a, b = fast.cpu().numpy(), sort.cpu().numpy()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 @pytest.mark.parametrize("shape", [(30, 40, 44), (16, 17, 50), (40, 70, 21)])
 def test_window_embedding_equals_full(gpu, shape):
     # The pruned path embeds only the atlas voxels the selection reads; with
