@@ -144,7 +144,7 @@ def batched_leg(steps: int, warmup: int, seed: int):
 
 def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup, seed,
                 plain_ms, train_shape=(64, 96, 96), train_subjects=4, train_rounds=2,
-                train_steps=60, topk=32, samples=27, margin=0.0, max_slabs=4, shape_id=-1):
+                train_steps=60, topk=32, samples=27, margin=0.0, max_slabs=3, shape_id=-1):
     """Learned-pruned fusion on the resident inputs: the embedding MLP is
     trained on the GPU on `train_subjects` other seeded phantoms, then the
     pruned fusion is timed and compared with the plain output."""

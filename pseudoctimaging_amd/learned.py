@@ -81,12 +81,14 @@ def bf16_words_to_float(words) -> np.ndarray:
 
 def fuse_pruned_device(img, atlas, atlas_label, out, nh: int, sigma: float, beta: float,
                        params: np.ndarray, topk: int = 32, samples: int = 27,
-                       margin: float = 0.0, shape_id: int = -1, max_slabs: int = 4,
+                       margin: float = 0.0, shape_id: int = -1, max_slabs: int = 3,
                        embed_all: bool = False, stream=None, report_fallback: bool = True):
     """Learned-pruned fusion on device tensors.  Returns the fraction of
     (tile, atlas) items that took the full search (synchronises) when
     report_fallback, else None (asynchronous).  max_slabs bounds the dz
-    planes the kept offsets come from (0: any)."""
+    planes the kept offsets come from (0: any).  The default 3 keeps them on
+    the lattice samples' planes: at 222x222x112, n=10, NH=11 it measured
+    faster than 4 (3.49 vs 3.80 ms) at the same error (1.90 HU MAE)."""
     params = _f32(params, "params")
     s = _stream(img, stream)
     pb = problem(img.shape, atlas.shape[0], nh, sigma, beta)
