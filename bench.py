@@ -228,6 +228,15 @@ def end_to_end_leg(img, amri, lab, nh, sigma, beta, reps):
         t_fuse.append(t1 - t0)
         t_post.append(t2 - t1)
     fuse_s, post_s = float(np.median(t_fuse)), float(np.median(t_post))
+    # the same region in one call, the post chain on the device
+    hu = pct.pseudo_ct_hu(img, amri, lab, nh, sigma, beta)   # warm-up
+    t_one = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        hu = pct.pseudo_ct_hu(img, amri, lab, nh, sigma, beta)
+        t_one.append(time.perf_counter() - t0)
+    one_s = float(np.median(t_one))
+    same = bool(np.array_equal(hu.view(np.uint32), fused.view(np.uint32)))
     # the reference's effective-bandwidth line (cu:2193-2196), on the whole region
     h = (nh - 1) // 2
     m = (nx - h) * (ny - h) * (nz - h)
@@ -237,10 +246,12 @@ def end_to_end_leg(img, amri, lab, nh, sigma, beta, reps):
     return {"fuse_host_ms": fuse_s * 1e3, "post_host_ms": post_s * 1e3,
             "total_ms": total * 1e3, "voxels_per_s": nx * ny * nz / total,
             "reference_effective_bandwidth_gbs": (rb + wb) / total / 1e9,
+            "pseudo_ct_host_ms": one_s * 1e3, "pseudo_ct_host_same_output": same,
             "pcie_bytes": 4 * nx * ny * nz * (2 + 2 * n) + 4 * nx * ny * nz,
             "reps": reps, "statistic": "median",
             "region": "pct_fuse_host (device alloc, 3 H2D, fusion, D2H, free) + host "
-                      "NaN fill, round and HU rescale"}
+                      "NaN fill, round and HU rescale; pseudo_ct_host_ms: the same region "
+                      "as pct_pseudo_ct_host, the post chain on the device"}
 
 
 def main() -> int:
@@ -404,7 +415,8 @@ def main() -> int:
         log("end-to-end leg: host buffers, per-call device memory, host post chain")
         e2e = end_to_end_leg(img, amri, lab, nh, sigma, beta, 3)
         log(f"end-to-end: {e2e['total_ms']:.1f} ms (fusion call {e2e['fuse_host_ms']:.1f} ms, "
-            f"post {e2e['post_host_ms']:.1f} ms)")
+            f"post {e2e['post_host_ms']:.1f} ms); pct_pseudo_ct_host "
+            f"{e2e['pseudo_ct_host_ms']:.1f} ms, same output {e2e['pseudo_ct_host_same_output']}")
 
     learned_res = None
     if not args.no_learned:

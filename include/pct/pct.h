@@ -93,6 +93,14 @@ int pct_normalize_labels_dev(float* d_label, size_t n, pct_stream_t stream);
 int pct_fuse_host(const pct_problem* pb, const float* img, const float* atlas,
                   const float* atlas_label_norm, float* out, int choice);
 
+/* The whole SM/GM/GM2 block of the reference's GPU main (cu:2152-2179): the
+ * fusion of pct_fuse_host, then RegularizationStep, RoundMatrix and the HU
+ * rescale (cpp:1059-1106, 1221-1227, cu:2166-2179) applied on the device
+ * before the single copy back.  `out_hu` equals pct_fuse_host followed by
+ * pct_postprocess bit for bit. */
+int pct_pseudo_ct_host(const pct_problem* pb, const float* img, const float* atlas,
+                       const float* atlas_label_norm, float* out_hu, int choice);
+
 /* Device-pointer variant: inputs already resident in HBM, enqueued on
  * `stream`, asynchronous.  Uses a per-device workspace owned by the library
  * (grown on demand; calls that overlap on one device must be ordered on one

@@ -10,7 +10,7 @@ from .api import (CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, autotune,  # noqa:
                   mat_read, mat_write, matrixio_read, matrixio_write, matrixio_write_sidecar,
                   multi_patch_based_segmentation_gpu,
                   normalize_labels, normalize_labels_device, postprocess, postprocess_device,
-                  problem, pseudo_ct, regularization_step_device,
+                  problem, pseudo_ct, pseudo_ct_hu, regularization_step_device,
                   read_config, regularization_step, round_matrix, synth_tissue, synth_uniform,
                   tile_config, to_hu, weight_divisor)
 
@@ -20,7 +20,7 @@ __all__ = [
     "force_tile_config", "fuse_device", "fused_to_hu_raw", "mat_read", "mat_write",
     "matrixio_read", "matrixio_write", "matrixio_write_sidecar",
     "multi_patch_based_segmentation_gpu", "normalize_labels", "normalize_labels_device",
-    "postprocess", "postprocess_device", "problem", "pseudo_ct", "read_config",
+    "postprocess", "postprocess_device", "problem", "pseudo_ct", "pseudo_ct_hu", "read_config",
     "regularization_step", "regularization_step_device", "round_matrix",
     "synth_tissue", "synth_uniform", "tile_config", "to_hu", "weight_divisor",
 ]

@@ -94,6 +94,7 @@ SIGNATURES = {
     "pct_normalize_labels": (None, [FP, c_size_t]),
     "pct_normalize_labels_dev": (c_int, [c_void_p, c_size_t, c_void_p]),
     "pct_fuse_host": (c_int, [POINTER(Problem), FP, FP, FP, FP, c_int]),
+    "pct_pseudo_ct_host": (c_int, [POINTER(Problem), FP, FP, FP, FP, c_int]),
     "pct_fuse_dev": (c_int, [POINTER(Problem), c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                              c_void_p]),
     "pct_fuse_workspace_size": (c_size_t, [POINTER(Problem), c_int]),

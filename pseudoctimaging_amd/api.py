@@ -85,6 +85,24 @@ def multi_patch_based_segmentation_gpu(img, atlas, atlas_label, nh: int, sigma: 
     return out
 
 
+def pseudo_ct_hu(img, atlas, atlas_label, nh: int, sigma: float = 1.0, beta: float = 1.0,
+                 choice: int = CHOICE_SM) -> np.ndarray:
+    """Final HU volume of the reference GPU program's SM block (cu:2152-2179):
+    fusion, NaN fill, rounding and HU rescale on the device, host arrays in
+    and out.  `atlas_label` is the normalised CT."""
+    img = _f32(img, "img")
+    atlas = _f32(atlas, "atlas")
+    atlas_label = _f32(atlas_label, "atlas_label")
+    if img.ndim != 3 or atlas.ndim != 4 or atlas.shape != atlas_label.shape \
+            or atlas.shape[1:] != img.shape:
+        raise ValueError("expected img (nz,nx,ny) and atlas/atlas_label (n,nz,nx,ny)")
+    pb = problem(img.shape, atlas.shape[0], nh, sigma, beta)
+    out = np.zeros(img.shape, np.float32)
+    check(lib().pct_pseudo_ct_host(byref(pb), fptr(img), fptr(atlas), fptr(atlas_label),
+                                   fptr(out), int(choice)), "pct_pseudo_ct_host")
+    return out
+
+
 def _dev_ptr(t, name: str):
     import torch
     if not isinstance(t, torch.Tensor) or t.device.type != "cuda":

@@ -144,10 +144,9 @@ int main(int argc, char** argv) {
 
     if (choice == PCT_CHOICE_GM || choice == PCT_CHOICE_GM2 || choice == PCT_CHOICE_SM) {
         // timed region as the reference's cu:2152-2188: fusion incl. copies,
-        // plus the host post-processing
+        // plus the post-processing (on the device, bit-identical to the host's)
         const double t0 = now_ms();
-        rc = pct_fuse_host(&pb, vs.img, vs.atlas, vs.label, out.data(), choice);
-        if (!rc) rc = pct_postprocess(nh, vs.nx, vs.ny, vs.nz, out.data());
+        rc = pct_pseudo_ct_host(&pb, vs.img, vs.atlas, vs.label, out.data(), choice);
         const double t1 = now_ms();
         if (rc) {
             std::fprintf(stderr, "fusion failed: %s\n", pct_strerror(rc));

@@ -665,8 +665,11 @@ extern "C" int pct_fuse_dev_ws(const pct_problem* pb, const float* d_img, const 
                          reinterpret_cast<hipStream_t>(stream));
 }
 
-extern "C" int pct_fuse_host(const pct_problem* pb, const float* img, const float* atlas,
-                             const float* label, float* out, int choice) {
+// Host buffers in and out around the device fusion (cu:1687-1744); with
+// `post`, the NaN fill, rounding and HU rescale run on the device before the
+// copy back.
+static int fuse_host_impl(const pct_problem* pb, const float* img, const float* atlas,
+                          const float* label, float* out, int choice, bool post) {
     int rc = validate_problem(pb);
     if (rc) return rc;
     if (!img || !atlas || !label || !out) return PCT_ERR_INVALID;
@@ -689,6 +692,9 @@ extern "C" int pct_fuse_host(const pct_problem* pb, const float* img, const floa
                 hipMemcpyAsync(d_label, label, NA * 4, hipMemcpyHostToDevice, s) != hipSuccess))
         rc = PCT_ERR_HIP;
     if (!rc) rc = fuse_dev_impl(pb, d_img, d_atlas, d_label, d_out, choice, nullptr, 0, true, s);
+    if (!rc && post)
+        rc = pct_postprocess_dev(pb->nh, pb->nx, pb->ny, pb->nz, d_out,
+                                 reinterpret_cast<pct_stream_t>(s));
     if (!rc && hipMemcpyAsync(out, d_out, N * 4, hipMemcpyDeviceToHost, s) != hipSuccess)
         rc = PCT_ERR_HIP;
     if (s) {
@@ -701,6 +707,16 @@ extern "C" int pct_fuse_host(const pct_problem* pb, const float* img, const floa
     if (d_label) (void)hipFree(d_label);
     if (d_out) (void)hipFree(d_out);
     return rc;
+}
+
+extern "C" int pct_fuse_host(const pct_problem* pb, const float* img, const float* atlas,
+                             const float* label, float* out, int choice) {
+    return fuse_host_impl(pb, img, atlas, label, out, choice, false);
+}
+
+extern "C" int pct_pseudo_ct_host(const pct_problem* pb, const float* img, const float* atlas,
+                                  const float* label, float* out_hu, int choice) {
+    return fuse_host_impl(pb, img, atlas, label, out_hu, choice, true);
 }
 
 extern "C" int pct_tile_config_count(void) { return (int)shapes().size(); }

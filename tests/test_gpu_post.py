@@ -92,3 +92,19 @@ def test_no_nan_and_empty_interior(gpu):
     assert _same(_gpu_post(a.copy(), 11), pct.postprocess(a.copy(), 11))
     b = rng.normal(100, 40, (5, 6, 7)).astype(np.float32)      # interior empty at NH=11
     assert _same(_gpu_post(b.copy(), 11), pct.postprocess(b.copy(), 11))
+
+
+@pytest.mark.parametrize("name", ["small_uniform.npz", "small_tissue.npz"])
+def test_pseudo_ct_host_equals_fusion_then_host_chain(gpu, name):
+    """pct_pseudo_ct_host (fusion and post chain on the device, one copy
+    back) against pct_fuse_host followed by the host chain."""
+    g = np.load(os.path.join(GOLDEN, name))
+    img, amri = g["img"], g["atlas"]
+    lab = np.array(g["ct_hu"], dtype=np.float32, copy=True)
+    pct.normalize_labels(lab)
+    nh, sigma, beta = int(g["params"][4]), float(g["params"][5]), float(g["params"][6])
+    fused = pct.multi_patch_based_segmentation_gpu(img, amri, lab, nh, sigma, beta)
+    print(f"case pseudo_ct_host {name}: {int(np.isnan(fused).sum())} NaN voxels", flush=True)
+    one = pct.pseudo_ct_hu(img, amri, lab, nh, sigma, beta)
+    two = pct.postprocess(fused, nh)
+    assert _same(one, two)
