@@ -97,7 +97,8 @@ int pct_fuse_host(const pct_problem* pb, const float* img, const float* atlas,
  * fusion of pct_fuse_host, then RegularizationStep, RoundMatrix and the HU
  * rescale (cpp:1059-1106, 1221-1227, cu:2166-2179) applied on the device
  * before the single copy back.  `out_hu` equals pct_fuse_host followed by
- * pct_postprocess bit for bit. */
+ * pct_postprocess bit for bit.  Volumes of nx*ny*nz >= 2^31 - 1 voxels
+ * return PCT_ERR_TOO_LARGE (the device post chain's limit). */
 int pct_pseudo_ct_host(const pct_problem* pb, const float* img, const float* atlas,
                        const float* atlas_label_norm, float* out_hu, int choice);
 
@@ -259,7 +260,8 @@ typedef struct pct_pruned_opts {
  * window sum times the mean exact weight of `samples` lattice offsets.  Tiles
  * with a low selection margin run the full exact search.  Output as
  * pct_fuse_dev.  If fallback_frac is not NULL the call synchronises and
- * stores the fraction of (tile, atlas) items that fell back. */
+ * stores the fraction of (tile, atlas) items that fell back.  Planes of
+ * nx*ny >= 2^31 voxels return PCT_ERR_TOO_LARGE. */
 int pct_fuse_pruned_dev(const pct_problem* pb, const float* params, const pct_pruned_opts* opts,
                         const float* d_img, const float* d_atlas,
                         const float* d_atlas_label_norm, float* d_out, double* fallback_frac,
@@ -330,7 +332,8 @@ int pct_postprocess(int nh, int nx, int ny, int nz, float* arr);
 /* Device versions of pct_regularize and pct_postprocess on a device volume,
  * enqueued on `stream`; bit-identical results.  The NaN fill visits the NaN
  * voxels in the reference's scan order, one at a time (cost grows with the
- * NaN count, not the volume).  Uses the library workspace. */
+ * NaN count, not the volume).  Uses the library workspace.  Volumes of
+ * nx*ny*nz >= 2^31 - 1 voxels return PCT_ERR_TOO_LARGE. */
 int pct_regularize_dev(int nh, int nx, int ny, int nz, float* d_arr, pct_stream_t stream);
 int pct_postprocess_dev(int nh, int nx, int ny, int nz, float* d_arr, pct_stream_t stream);
 

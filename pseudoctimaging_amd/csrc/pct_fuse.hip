@@ -673,9 +673,10 @@ static int fuse_host_impl(const pct_problem* pb, const float* img, const float* 
     int rc = validate_problem(pb);
     if (rc) return rc;
     if (!img || !atlas || !label || !out) return PCT_ERR_INVALID;
+    const size_t N = (size_t)pb->nx * pb->ny * pb->nz;
+    if (post && N >= (size_t)INT_MAX) return PCT_ERR_TOO_LARGE;   // device post chain's limit
     rc = check_device(nullptr);
     if (rc) return rc;
-    const size_t N = (size_t)pb->nx * pb->ny * pb->nz;
     const size_t NA = N * pb->n_atlas;
     float *d_img = nullptr, *d_atlas = nullptr, *d_label = nullptr, *d_out = nullptr;
     hipStream_t s = nullptr;

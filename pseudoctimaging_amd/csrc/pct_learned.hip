@@ -1314,6 +1314,8 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     if (opts->topk < 1 || opts->topk > kMaxTopk || opts->samples < 0) return PCT_ERR_INVALID;
     const int hw = (pb->nh - 1) / 2;
     if (hw < 1 || hw > kMaxTiledHalfWindow) return PCT_ERR_UNSUPPORTED;
+    // the direct LDS staging keeps in-plane source offsets in 32 bits
+    if ((long long)pb->nx * pb->ny >= INT_MAX) return PCT_ERR_TOO_LARGE;
     int dev = 0;
     rc = check_device(&dev);
     if (rc) return rc;
