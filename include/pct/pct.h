@@ -85,11 +85,15 @@ int pct_normalize_labels_dev(float* d_label, size_t n, pct_stream_t stream);
 
 /* Replaces MultiPatchBasedSegmentationGpu (src/GPU/PatchBasedPseudoCT.cu:1635)
  * and its CPU twin MultiAtlasPatchBasedSegmentationOpenMP (cpp:1350).
- * Host buffers; allocates device memory, copies in, runs, copies `out` back
- * and frees, like the reference wrapper (cu:1687-1744).  `out` receives every
- * voxel: interior = fused value (NaN where |P| or |S| <= DBL_EPSILON,
- * cpp:1437-1443), border = 0 (cu:1693).  `choice` is a pct_exec_choice
- * (GM, GM2 or SM). */
+ * Host buffers; copies in, runs and copies `out` back like the reference
+ * wrapper (cu:1687-1744), and returns when `out` is written.  The atlas
+ * copies go in chunks that overlap the fusion of the chunks already copied.
+ * The device buffers stay allocated for the next host-buffer call (grown on
+ * demand; pct_release frees them); host-buffer calls on one device run one
+ * at a time.  `out` receives every voxel: interior = fused value (NaN where
+ * |P| or |S| <= DBL_EPSILON, cpp:1437-1443), border = 0 (cu:1693).
+ * `choice` is a pct_exec_choice (GM, GM2 or SM).  The output equals
+ * pct_fuse_dev's bit for bit. */
 int pct_fuse_host(const pct_problem* pb, const float* img, const float* atlas,
                   const float* atlas_label_norm, float* out, int choice);
 

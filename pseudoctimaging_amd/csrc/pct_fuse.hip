@@ -665,50 +665,176 @@ extern "C" int pct_fuse_dev_ws(const pct_problem* pb, const float* d_img, const 
                          reinterpret_cast<hipStream_t>(stream));
 }
 
+// Device buffers, streams and events of the host-buffer entry points, kept
+// per device between calls (grown on demand, freed by pct_release).  One
+// host-buffer call at a time runs on a device.
+constexpr int kCopyChunks = 10;  // atlas chunks whose copies overlap the fusion
+struct HostCallState {
+    std::mutex mu;
+    float *img = nullptr, *atlas = nullptr, *label = nullptr, *out = nullptr;
+    size_t n_cap = 0, na_cap = 0;
+    // chunk kernels alternate between two compute streams, so that the tail
+    // of one chunk overlaps the start of the next
+    hipStream_t compute = nullptr, compute2 = nullptr, copy = nullptr;
+    hipEvent_t ev[kCopyChunks] = {};
+    hipEvent_t join = nullptr;
+};
+static std::mutex g_host_mutex;
+static std::map<int, HostCallState*> g_host_state;
+
+static HostCallState& host_state(int dev) {
+    std::lock_guard<std::mutex> lk(g_host_mutex);
+    HostCallState*& st = g_host_state[dev];
+    if (!st) st = new HostCallState();
+    return *st;
+}
+
+static void free_host_buffers(HostCallState& st) {
+    if (st.img) (void)hipFree(st.img);
+    if (st.atlas) (void)hipFree(st.atlas);
+    if (st.label) (void)hipFree(st.label);
+    if (st.out) (void)hipFree(st.out);
+    st.img = st.atlas = st.label = st.out = nullptr;
+    st.n_cap = st.na_cap = 0;
+}
+
+static int prepare_host_state(HostCallState& st, size_t N, size_t NA) {
+    if (!st.compute) {
+        if (hipStreamCreateWithFlags(&st.compute, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&st.compute2, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&st.copy, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&st.join, hipEventDisableTiming) != hipSuccess)
+            return PCT_ERR_HIP;
+        for (auto& e : st.ev)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+                return PCT_ERR_HIP;
+    }
+    if (st.n_cap < N || st.na_cap < NA) {
+        free_host_buffers(st);
+        if (hipMalloc(&st.img, N * 4) != hipSuccess || hipMalloc(&st.atlas, NA * 4) != hipSuccess ||
+            hipMalloc(&st.label, NA * 4) != hipSuccess || hipMalloc(&st.out, N * 4) != hipSuccess) {
+            (void)hipGetLastError();
+            free_host_buffers(st);
+            return PCT_ERR_NO_MEMORY;
+        }
+        st.n_cap = N;
+        st.na_cap = NA;
+    }
+    return PCT_OK;
+}
+
 // Host buffers in and out around the device fusion (cu:1687-1744); with
 // `post`, the NaN fill, rounding and HU rescale run on the device before the
-// copy back.
+// copy back.  For the tiled kernel the atlases are copied in chunks of whole
+// atlas groups on a copy stream, and each chunk's work items run on one of
+// two compute streams as soon as its copy has landed, so the copies of the
+// later chunks overlap the fusion of the earlier ones.  Every item writes the same
+// partial sums as in one launch, and the finalize pass adds the groups in
+// their fixed order: the output is that of pct_fuse_dev bit for bit.
 static int fuse_host_impl(const pct_problem* pb, const float* img, const float* atlas,
                           const float* label, float* out, int choice, bool post) {
     int rc = validate_problem(pb);
     if (rc) return rc;
     if (!img || !atlas || !label || !out) return PCT_ERR_INVALID;
+    if (choice != PCT_CHOICE_GM && choice != PCT_CHOICE_GM2 && choice != PCT_CHOICE_SM)
+        return PCT_ERR_INVALID;
     const size_t N = (size_t)pb->nx * pb->ny * pb->nz;
     if (post && N >= (size_t)INT_MAX) return PCT_ERR_TOO_LARGE;   // device post chain's limit
-    rc = check_device(nullptr);
+    int dev = 0;
+    rc = check_device(&dev);
     if (rc) return rc;
     const size_t NA = N * pb->n_atlas;
-    float *d_img = nullptr, *d_atlas = nullptr, *d_label = nullptr, *d_out = nullptr;
-    hipStream_t s = nullptr;
-    rc = PCT_OK;
-    // cu:1687-1699: device allocation and host->device copies per call
-    if (hipMalloc(&d_img, N * 4) != hipSuccess || hipMalloc(&d_atlas, NA * 4) != hipSuccess ||
-        hipMalloc(&d_label, NA * 4) != hipSuccess || hipMalloc(&d_out, N * 4) != hipSuccess) {
-        (void)hipGetLastError();
-        rc = PCT_ERR_NO_MEMORY;
-    }
-    if (!rc && (hipStreamCreate(&s) != hipSuccess)) rc = PCT_ERR_HIP;
-    if (!rc && (hipMemcpyAsync(d_img, img, N * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
-                hipMemcpyAsync(d_atlas, atlas, NA * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
-                hipMemcpyAsync(d_label, label, NA * 4, hipMemcpyHostToDevice, s) != hipSuccess))
+    HostCallState& st = host_state(dev);
+    std::lock_guard<std::mutex> lk(st.mu);
+    rc = prepare_host_state(st, N, NA);
+    if (rc) return rc;
+    hipStream_t s = st.compute, cs = st.copy;
+    int sid = 0, grp = 1;
+    TileGeometry g;
+    const bool tiled = is_tiled_choice(pb, choice) && choose(pb, &sid, &grp, nullptr, &g);
+    void* ws = nullptr;
+    if (tiled) rc = ensure_workspace(dev, tiled_ws_bytes(pb, g.n_groups), &ws);
+    // cu:1687-1699: host->device copies
+    if (!rc && hipMemcpyAsync(st.img, img, N * 4, hipMemcpyHostToDevice, cs) != hipSuccess)
         rc = PCT_ERR_HIP;
-    if (!rc) rc = fuse_dev_impl(pb, d_img, d_atlas, d_label, d_out, choice, nullptr, 0, true, s);
+    if (!rc && tiled) {
+        const int chunks = std::min(g.n_groups, kCopyChunks);
+        for (int c = 0; c < chunks && !rc; ++c) {
+            const int g0 = (int)((long long)c * g.n_groups / chunks);
+            const int g1 = (int)((long long)(c + 1) * g.n_groups / chunks);
+            const int a0 = g0 * g.group, a1 = std::min(pb->n_atlas, g1 * g.group);
+            const size_t off = (size_t)a0 * N, len = (size_t)(a1 - a0) * N;
+            if (hipMemcpyAsync(st.atlas + off, atlas + off, len * 4, hipMemcpyHostToDevice, cs) !=
+                    hipSuccess ||
+                hipMemcpyAsync(st.label + off, label + off, len * 4, hipMemcpyHostToDevice, cs) !=
+                    hipSuccess ||
+                hipEventRecord(st.ev[c], cs) != hipSuccess ||
+                hipStreamWaitEvent((c & 1) ? st.compute2 : s, st.ev[c], 0) != hipSuccess) {
+                rc = PCT_ERR_HIP;
+                break;
+            }
+            // the items of groups g0 .. g1-1: pointers offset to the chunk
+            TiledArgs a = make_tiled_args(pb, g, st.img, st.atlas + off, st.label + off,
+                                          static_cast<float*>(ws) + 2 * (size_t)g0 * N);
+            a.n_atlas = a1 - a0;
+            a.n_items = (int)(g.n_tiles * (g1 - g0));
+            rc = launch_tiled_items(a, sid, g.hw, (c & 1) ? st.compute2 : s);
+        }
+        if (!rc && (hipEventRecord(st.join, st.compute2) != hipSuccess ||
+                    hipStreamWaitEvent(s, st.join, 0) != hipSuccess))
+            rc = PCT_ERR_HIP;
+        if (!rc) rc = launch_finalize(pb, g, static_cast<const float*>(ws), st.out, s);
+    } else if (!rc) {
+        if (hipMemcpyAsync(st.atlas, atlas, NA * 4, hipMemcpyHostToDevice, cs) != hipSuccess ||
+            hipMemcpyAsync(st.label, label, NA * 4, hipMemcpyHostToDevice, cs) != hipSuccess ||
+            hipEventRecord(st.ev[0], cs) != hipSuccess ||
+            hipStreamWaitEvent(s, st.ev[0], 0) != hipSuccess)
+            rc = PCT_ERR_HIP;
+        if (!rc) rc = fuse_dev_impl(pb, st.img, st.atlas, st.label, st.out, choice, nullptr, 0,
+                                    true, s);
+    }
     if (!rc && post)
-        rc = pct_postprocess_dev(pb->nh, pb->nx, pb->ny, pb->nz, d_out,
+        rc = pct_postprocess_dev(pb->nh, pb->nx, pb->ny, pb->nz, st.out,
                                  reinterpret_cast<pct_stream_t>(s));
-    if (!rc && hipMemcpyAsync(out, d_out, N * 4, hipMemcpyDeviceToHost, s) != hipSuccess)
+    if (!rc && hipMemcpyAsync(out, st.out, N * 4, hipMemcpyDeviceToHost, s) != hipSuccess)
         rc = PCT_ERR_HIP;
-    if (s) {
-        if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = PCT_ERR_HIP;
-        (void)hipStreamDestroy(s);
-    }
-    // cu:1744
-    if (d_img) (void)hipFree(d_img);
-    if (d_atlas) (void)hipFree(d_atlas);
-    if (d_label) (void)hipFree(d_label);
-    if (d_out) (void)hipFree(d_out);
+    // every stream is drained before returning, also after an error
+    if (hipStreamSynchronize(cs) != hipSuccess && !rc) rc = PCT_ERR_HIP;
+    if (hipStreamSynchronize(st.compute2) != hipSuccess && !rc) rc = PCT_ERR_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = PCT_ERR_HIP;
+    if (rc) (void)hipGetLastError();
     return rc;
 }
+
+namespace pct {
+void release_host_buffers() {
+    std::lock_guard<std::mutex> lk(g_host_mutex);
+    for (auto& kv : g_host_state) {
+        HostCallState* st = kv.second;
+        {
+            // waits for a call still running on this device; released
+            // before the state (and its mutex) is deleted
+            std::lock_guard<std::mutex> lk2(st->mu);
+            int cur = 0;
+            (void)hipGetDevice(&cur);
+            (void)hipSetDevice(kv.first);
+            if (st->compute) (void)hipStreamSynchronize(st->compute);
+            if (st->compute2) (void)hipStreamSynchronize(st->compute2);
+            if (st->copy) (void)hipStreamSynchronize(st->copy);
+            free_host_buffers(*st);
+            for (auto& e : st->ev)
+                if (e) (void)hipEventDestroy(e);
+            if (st->join) (void)hipEventDestroy(st->join);
+            if (st->compute) (void)hipStreamDestroy(st->compute);
+            if (st->compute2) (void)hipStreamDestroy(st->compute2);
+            if (st->copy) (void)hipStreamDestroy(st->copy);
+            (void)hipSetDevice(cur);
+        }
+        delete st;
+    }
+    g_host_state.clear();
+}
+}  // namespace pct
 
 extern "C" int pct_fuse_host(const pct_problem* pb, const float* img, const float* atlas,
                              const float* label, float* out, int choice) {
@@ -839,6 +965,7 @@ extern "C" int pct_autotune(const pct_problem* pb, const float* d_img, const flo
 extern "C" void pct_release(void) {
     release_batch_graphs();
     release_learned_models();
+    release_host_buffers();
     std::lock_guard<std::mutex> lk(g_dev_mutex);
     for (auto& kv : g_dev) {
         if (kv.second.ws) {

@@ -187,6 +187,16 @@ extern "C" int pct_regularize(int nh, int nx, int ny, int nz, float* arr) {
     if (!arr || nh < 1 || nx < 1 || ny < 1 || nz < 1) return PCT_ERR_INVALID;
     const int hp = 1, hw = (nh - 1) / 2, lo = hw + hp;
     const long long pl = (long long)nx * ny;
+    // A volume without an interior NaN leaves the sweep after one pass that
+    // changes nothing; find that out in parallel first.
+    int any_nan = 0;
+#pragma omp parallel for schedule(static) reduction(| : any_nan)
+    for (int z = lo; z < nz - lo; ++z)
+        for (int x = lo; x < nx - lo; ++x) {
+            const float* row = arr + z * pl + (long long)x * ny;
+            for (int y = lo; y < ny - lo; ++y) any_nan |= std::isnan(row[y]) ? 1 : 0;
+        }
+    if (!any_nan) return 1;
     int passes = 0;
     bool again = true;
     while (again) {
@@ -215,23 +225,30 @@ extern "C" int pct_regularize(int nh, int nx, int ny, int nz, float* arr) {
     return passes;
 }
 
+// Both are elementwise, so the parallel loops give the sequential result.
 extern "C" void pct_round(float* arr, size_t n) {
     if (!arr) return;
-    for (size_t i = 0; i < n; ++i) arr[i] = std::round(arr[i]);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; ++i) arr[i] = std::round(arr[i]);
 }
 
 extern "C" void pct_to_hu(float* arr, size_t n) {
     if (!arr) return;
     const float max_labels = 4095.0f, min_labels = 1024.0f;
-    for (size_t i = 0; i < n; ++i) arr[i] = std::round(max_labels * (arr[i] / 255.0f)) - min_labels;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; ++i)
+        arr[i] = std::round(max_labels * (arr[i] / 255.0f)) - min_labels;
 }
 
 extern "C" int pct_postprocess(int nh, int nx, int ny, int nz, float* arr) {
     int rc = pct_regularize(nh, nx, ny, nz, arr);
     if (rc < 0) return rc;
-    const size_t n = (size_t)nx * ny * nz;
-    pct_round(arr, n);
-    pct_to_hu(arr, n);
+    // RoundMatrix then the HU rescale, in one pass with the same operations
+    const long long n = (long long)nx * ny * nz;
+    const float max_labels = 4095.0f, min_labels = 1024.0f;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n; ++i)
+        arr[i] = std::round(max_labels * (std::round(arr[i]) / 255.0f)) - min_labels;
     return PCT_OK;
 }
 

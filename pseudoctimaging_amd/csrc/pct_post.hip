@@ -6,8 +6,8 @@
 // RegularizationStep sweeps the interior in z, x, y order and overwrites each
 // NaN voxel in place, so a voxel sees the values filled earlier in the same
 // sweep.  Only NaN voxels ever change, so the sweep is reproduced exactly by
-// (1) listing the interior NaN voxels in scan order (one workgroup, ordered
-// prefix sums) and (2) filling them in list order with one wave (the 27
+// (1) listing the interior NaN voxels in scan order (per-chunk counts, a
+// scan, an ordered compaction per chunk) and (2) filling them in list order with one wave (the 27
 // neighbours on 27 lanes, the median by stable ranks across lanes); voxels
 // whose median is still NaN form the next sweep's list.  Sweeps stop when no
 // NaN is left or a sweep fills nothing.  Rounding and the rescale are
@@ -23,45 +23,103 @@
 namespace pct {
 
 // Interior NaN voxels of `arr` in scan order (z, x, y; y fastest) into
-// list[0..*count).  One workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void nan_list_kernel(const float* __restrict__ arr, int nx,
-                                                        int ny, int nz, int lo,
-                                                        int* __restrict__ list,
-                                                        int* __restrict__ count) {
-    __shared__ int wave_cnt[16];
-    __shared__ int base;
-    if (threadIdx.x == 0) base = 0;
+// list[0..*count), in three passes over chunks of kListChunk interior
+// positions (one block each): count per chunk, exclusive scan of the counts,
+// then an ordered compaction of every chunk at its offset.
+constexpr int kListThreads = 256;
+constexpr int kListChunk = 4096;
+
+__device__ __forceinline__ int interior_id(long long t, int lo, int ix, int iy, int nx, int ny) {
+    const int y = lo + (int)(t % iy);
+    const int x = lo + (int)((t / iy) % ix);
+    const int z = lo + (int)(t / ((long long)iy * ix));
+    return (z * nx + x) * ny + y;
+}
+
+__global__ __launch_bounds__(kListThreads) void nan_count_kernel(const float* __restrict__ arr,
+                                                                 int nx, int ny, int lo, int ix,
+                                                                 int iy, long long total,
+                                                                 int* __restrict__ bcount) {
+    __shared__ int wsum[kListThreads / 64];
+    const long long c0 = (long long)blockIdx.x * kListChunk;
+    int cnt = 0;
+    for (int k = threadIdx.x; k < kListChunk; k += kListThreads) {
+        const long long t = c0 + k;
+        if (t < total && isnan(arr[interior_id(t, lo, ix, iy, nx, ny)])) ++cnt;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = cnt;
     __syncthreads();
-    const int iz = nz - 2 * lo, ix = nx - 2 * lo, iy = ny - 2 * lo;
-    const long long total = (iz > 0 && ix > 0 && iy > 0) ? (long long)iz * ix * iy : 0;
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < kListThreads / 64; ++w) tot += wsum[w];
+        bcount[blockIdx.x] = tot;
+    }
+}
+
+// Exclusive scan of nb block counts into boff; the total into *count.
+__global__ __launch_bounds__(1024) void nan_scan_kernel(const int* __restrict__ bcount, int nb,
+                                                        int* __restrict__ boff,
+                                                        int* __restrict__ count) {
+    __shared__ int wtot[16];
+    const int seg = (nb + 1023) / 1024;
+    const int b0 = threadIdx.x * seg, b1 = min(nb, b0 + seg);
+    int own = 0;
+    for (int b = b0; b < b1; ++b) own += bcount[b];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (long long c0 = 0; c0 < total; c0 += 1024) {
-        const long long t = c0 + threadIdx.x;
-        bool isn = false;
+    int incl = own;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) wtot[w] = incl;
+    __syncthreads();
+    int before = 0;
+    for (int j = 0; j < w; ++j) before += wtot[j];
+    int run = before + incl - own;   // exclusive prefix of this thread's segment
+    for (int b = b0; b < b1; ++b) {
+        boff[b] = run;
+        run += bcount[b];
+    }
+    if (threadIdx.x == 1023) {
+        int tot = 0;
+        for (int j = 0; j < 16; ++j) tot += wtot[j];
+        *count = tot;
+    }
+}
+
+__global__ __launch_bounds__(kListThreads) void nan_write_kernel(const float* __restrict__ arr,
+                                                                 int nx, int ny, int lo, int ix,
+                                                                 int iy, long long total,
+                                                                 const int* __restrict__ boff,
+                                                                 int* __restrict__ list) {
+    __shared__ int wcnt[kListThreads / 64];
+    const long long c0 = (long long)blockIdx.x * kListChunk;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int base = boff[blockIdx.x];
+    for (int r = 0; r < kListChunk; r += kListThreads) {
+        const long long t = c0 + r + threadIdx.x;
         int id = 0;
+        bool isn = false;
         if (t < total) {
-            const int y = lo + (int)(t % iy);
-            const int x = lo + (int)((t / iy) % ix);
-            const int z = lo + (int)(t / ((long long)iy * ix));
-            id = (z * nx + x) * ny + y;
+            id = interior_id(t, lo, ix, iy, nx, ny);
             isn = isnan(arr[id]);
         }
         const unsigned long long bal = __ballot(isn);
-        const int before = __popcll(bal & ((1ull << lane) - 1ull));
-        if (lane == 0) wave_cnt[w] = __popcll(bal);
+        if (lane == 0) wcnt[w] = __popcll(bal);
         __syncthreads();
-        int off = base;
-        for (int j = 0; j < w; ++j) off += wave_cnt[j];
-        if (isn) list[off + before] = id;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int tot = 0;
-            for (int j = 0; j < 16; ++j) tot += wave_cnt[j];
-            base += tot;
+        int off = base, tot = 0;
+#pragma unroll
+        for (int j = 0; j < kListThreads / 64; ++j) {
+            if (j < w) off += wcnt[j];
+            tot += wcnt[j];
         }
-        __syncthreads();
+        if (isn) list[off + __popcll(bal & ((1ull << lane) - 1ull))] = id;
+        __syncthreads();   // wcnt is rewritten next round
+        base += tot;
     }
-    if (threadIdx.x == 0) *count = base;
 }
 
 // Value of the valid lane whose stable rank is `pos` (all lanes receive it).
@@ -159,15 +217,23 @@ static int enqueue_fill(int nh, int nx, int ny, int nz, float* d_arr, hipStream_
     const long long interior = (iz > 0 && ix > 0 && iy > 0) ? iz * ix * iy : 0;
     if (interior == 0) return PCT_OK;
     const size_t list_bytes = ((size_t)interior * 4 + 255) & ~(size_t)255;
+    const long long nb = (interior + kListChunk - 1) / kListChunk;
+    const size_t block_bytes = ((size_t)nb * 4 + 255) & ~(size_t)255;
     void* ws = nullptr;
-    rc = ensure_workspace(dev, 2 * list_bytes + 256, &ws);
+    rc = ensure_workspace(dev, 256 + 2 * list_bytes + 2 * block_bytes, &ws);
     if (rc) return rc;
     char* w = static_cast<char*>(ws);
     int* count = reinterpret_cast<int*>(w);
     int* la = reinterpret_cast<int*>(w + 256);
     int* lb = reinterpret_cast<int*>(w + 256 + list_bytes);
-    hipLaunchKernelGGL(nan_list_kernel, dim3(1), dim3(1024), 0, s, d_arr, nx, ny, nz, lo, la,
-                       count);
+    int* bcount = reinterpret_cast<int*>(w + 256 + 2 * list_bytes);
+    int* boff = reinterpret_cast<int*>(w + 256 + 2 * list_bytes + block_bytes);
+    const int ixi = (int)ix, iyi = (int)iy;
+    hipLaunchKernelGGL(nan_count_kernel, dim3((unsigned)nb), dim3(kListThreads), 0, s, d_arr, nx,
+                       ny, lo, ixi, iyi, interior, bcount);
+    hipLaunchKernelGGL(nan_scan_kernel, dim3(1), dim3(1024), 0, s, bcount, (int)nb, boff, count);
+    hipLaunchKernelGGL(nan_write_kernel, dim3((unsigned)nb), dim3(kListThreads), 0, s, d_arr, nx,
+                       ny, lo, ixi, iyi, interior, boff, la);
     hipLaunchKernelGGL(nan_fill_kernel, dim3(1), dim3(64), 0, s, d_arr, la, lb, count, nx, ny,
                        (int*)nullptr);
     return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;

@@ -453,5 +453,9 @@ void release_batch_graphs();
 // Frees the per-device copies of the embedding network (pct_learned.hip).
 void release_learned_models();
 
+// Frees the device buffers, streams and events of the host-buffer entry
+// points (pct_fuse.hip).
+void release_host_buffers();
+
 }  // namespace pct
 

@@ -168,6 +168,41 @@ def test_uniform_random_mode_with_nan(gpu):
             print(sigma, choice, st)
 
 
+@pytest.mark.parametrize("case", [
+    # nx, ny, nz, n, nh, forced (shape, group) or None, choice
+    (40, 66, 30, 10, 9, None, pct.CHOICE_SM),         # 4 chunks of uneven group counts
+    (36, 40, 28, 3, 11, (7, 1), pct.CHOICE_SM),       # 3 chunks
+    (36, 40, 28, 10, 7, (11, 3), pct.CHOICE_SM),      # groups of 3 atlases, the last short
+    (30, 34, 26, 5, 11, (13, 5), pct.CHOICE_SM),      # one group: one chunk
+    (30, 34, 26, 2, 17, None, pct.CHOICE_SM),         # no tiled kernel: direct path
+    (30, 34, 26, 2, 9, None, pct.CHOICE_GM2),
+])
+def test_host_call_overlapped_copies_equal_device_call(gpu, case):
+    """pct_fuse_host copies the atlases in chunks that overlap the fusion; its
+    output equals pct_fuse_dev's bit for bit, also across calls that reuse
+    the cached device buffers with smaller and larger problems."""
+    import torch
+    nx, ny, nz, n, nh, forced, choice = case
+    print(f"case host overlap {case}", flush=True)
+    dev = torch.device("cuda:0")
+    try:
+        if forced:
+            pct.force_tile_config(*forced)
+        for k, scale in enumerate((1.0, 0.5, 1.0)):
+            sx, sy = max(12, int(nx * scale)), max(12, int(ny * scale))
+            img, amri, act = pct.synth_tissue(sx, sy, nz, n, 2, False, 40 + k)
+            lab = act.copy()
+            pct.normalize_labels(lab)
+            host = pct.multi_patch_based_segmentation_gpu(img, amri, lab, nh, 2.5, 1.0, choice)
+            out = torch.empty(img.shape, device=dev)
+            pct.fuse_device(torch.from_numpy(img).to(dev), torch.from_numpy(amri).to(dev),
+                            torch.from_numpy(lab).to(dev), out, nh, 2.5, 1.0, choice)
+            torch.cuda.synchronize()
+            assert np.array_equal(out.cpu().numpy().view(np.uint32), host.view(np.uint32)), k
+    finally:
+        pct.force_tile_config(-1, 0)
+
+
 def test_device_entry_points(gpu):
     import torch
     img, amri, act = pct.synth_tissue(40, 66, 30, 3, 2, False, 21)
