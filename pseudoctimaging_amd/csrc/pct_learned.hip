@@ -10,7 +10,8 @@
 //                 16x16 output tile of one layer is, lane for lane, the B
 //                 operand of the next layer, so no data moves between layers;
 //                 the 28 weight fragments stay in VGPRs.
-//  box_pass       NH-wide box sums of the atlas CT (three separable passes).
+//  box_xy, box_z  NH-wide box sums of the atlas CT (an LDS pass over y and x,
+//                 running sums over z).
 //  select_kernel  per (tile, atlas): embedding distance of every window
 //                 offset summed over a 4x2x2 probe cluster at the tile centre
 //                 (probe-by-voxel dot products on BF16 MFMA, one window
@@ -552,89 +553,80 @@ static int launch_embed_windows(const float* params, const void* d_model, const 
 // window box sums of the atlas CT
 // ---------------------------------------------------------------------------
 
-// out = sum of in over [c - hw, c + hw] along `axis` (0 z, 1 x, 2 y), zero
-// outside the volume; nvol volumes one after the other.
-__global__ void box_pass(const float* __restrict__ in, float* __restrict__ out, int nx, int ny,
-                         int nz, int nvol, int hw, int axis) {
-    const long long vsize = (long long)nx * ny * nz;
-    const long long total = vsize * nvol;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
-         i += (long long)gridDim.x * blockDim.x) {
-        const long long vi = i / vsize;
-        const long long r = i - vi * vsize;
-        const int y = (int)(r % ny);
-        const int x = (int)((r / ny) % nx);
-        const int z = (int)(r / ((long long)nx * ny));
-        const float* __restrict__ v = in + vi * vsize;
-        float s = 0.f;
-        if (axis == 2) {
-            const int lo = max(0, y - hw), hi = min(ny - 1, y + hw);
-            for (int k = lo; k <= hi; ++k) s += v[((long long)z * nx + x) * ny + k];
-        } else if (axis == 1) {
-            const int lo = max(0, x - hw), hi = min(nx - 1, x + hw);
-            for (int k = lo; k <= hi; ++k) s += v[((long long)z * nx + k) * ny + y];
-        } else {
-            const int lo = max(0, z - hw), hi = min(nz - 1, z + hw);
-            for (int k = lo; k <= hi; ++k) s += v[((long long)k * nx + x) * ny + y];
-        }
-        out[i] = s;
-    }
-}
+// The sums along y and x in one pass: a block stages a (16 + 2 hw) x
+// (64 + 2 hw) tile of one plane in LDS (zeros outside the volume), sums
+// each row over y, then the row sums over x.
+constexpr int kBoxTX = 16, kBoxTY = 64;
 
-// The same sums along y: one wave per row, the row staged in LDS with hw
-// zeros on each side, outputs summed from LDS in the order of box_pass.
-__global__ __launch_bounds__(256) void box_y(const float* __restrict__ in,
-                                             float* __restrict__ out, int ny, long long rows,
-                                             int hw) {
+__global__ __launch_bounds__(256) void box_xy(const float* __restrict__ in,
+                                              float* __restrict__ out, int nx, int ny,
+                                              long long planes, int hw) {
     extern __shared__ float sh[];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int rl = ny + 2 * hw;
-    float* R = sh + wv * rl;
-    const long long row = (long long)blockIdx.x * 4 + wv;
-    const bool active = row < rows;
-    if (active) {
-        const float* __restrict__ src = in + row * ny;
-        for (int c = lane; c < rl; c += 64) {
-            const int y = c - hw;
-            R[c] = (y >= 0 && y < ny) ? src[y] : 0.f;
-        }
+    const int rows = kBoxTX + 2 * hw, cols = kBoxTY + 2 * hw;
+    float* T = sh;                  // [rows][cols]
+    float* Y = sh + rows * cols;    // [rows][kBoxTY] sums over y
+    const int tiles_x = (nx + kBoxTX - 1) / kBoxTX, tiles_y = (ny + kBoxTY - 1) / kBoxTY;
+    const long long b = blockIdx.x;
+    const int ty = (int)(b % tiles_y);
+    const int tx = (int)((b / tiles_y) % tiles_x);
+    const long long pl = b / ((long long)tiles_y * tiles_x);
+    if (pl >= planes) return;   // uniform over the block
+    const int x0 = tx * kBoxTX, y0 = ty * kBoxTY;
+    const float* __restrict__ src = in + pl * nx * (long long)ny;
+    for (int i = threadIdx.x; i < rows * cols; i += 256) {
+        const int r = i / cols, c = i - (i / cols) * cols;
+        const int x = x0 - hw + r, y = y0 - hw + c;
+        T[i] = (x >= 0 && x < nx && y >= 0 && y < ny) ? src[(long long)x * ny + y] : 0.f;
     }
     __syncthreads();
-    if (!active) return;
-    float* __restrict__ dst = out + row * ny;
-    for (int y = lane; y < ny; y += 64) {
+    for (int i = threadIdx.x; i < rows * kBoxTY; i += 256) {
+        const int r = i / kBoxTY, c = i - (i / kBoxTY) * kBoxTY;
         float acc = 0.f;
-        for (int k = 0; k <= 2 * hw; ++k) acc += R[y + k];
-        dst[y] = acc;
+        for (int k = 0; k <= 2 * hw; ++k) acc += T[r * cols + c + k];
+        Y[i] = acc;
+    }
+    __syncthreads();
+    float* __restrict__ dst = out + pl * nx * (long long)ny;
+    for (int i = threadIdx.x; i < kBoxTX * kBoxTY; i += 256) {
+        const int r = i / kBoxTY, c = i - (i / kBoxTY) * kBoxTY;
+        const int x = x0 + r, y = y0 + c;
+        if (x >= nx || y >= ny) continue;
+        float acc = 0.f;
+        for (int k = 0; k <= 2 * hw; ++k) acc += Y[(r + k) * kBoxTY + c];
+        dst[(long long)x * ny + y] = acc;
     }
 }
 
-// The same sums along x (axis 1) or z (axis 0) as a running sum in double:
-// one thread per line, the lanes of a wave on consecutive y (coalesced), two
-// reads per output instead of 2 hw + 1.
-__global__ void box_line(const float* __restrict__ in, float* __restrict__ out, int nx, int ny,
-                         int nz, int nvol, int hw, int axis) {
-    const int other = axis == 1 ? nz : nx;       // the line's remaining coordinate
-    const long long lines = (long long)nvol * other * ny;
+// The sums along z as running sums in double over segments of kBoxSeg
+// planes: one thread per (line, segment), the lanes of a wave on
+// consecutive y.  A segment starts from the direct sum of its first window.
+constexpr int kBoxSeg = 16;
+
+__global__ void box_z(const float* __restrict__ in, float* __restrict__ out, int nx, int ny,
+                      int nz, int nvol, int hw) {
+    const int segs = (nz + kBoxSeg - 1) / kBoxSeg;
+    const long long work = (long long)nvol * nx * ny * segs;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= lines) return;
+    if (t >= work) return;
     const int y = (int)(t % ny);
-    const long long rr = t / ny;
-    const int o = (int)(rr % other);
-    const long long vi = rr / other;
-    const int len = axis == 1 ? nx : nz;
-    const long long stride = axis == 1 ? (long long)ny : (long long)nx * ny;
-    const long long base = vi * nx * ny * (long long)nz +
-                           (axis == 1 ? (long long)o * nx * ny : (long long)o * ny) + y;
+    long long rr = t / ny;
+    const int sgi = (int)(rr % segs);
+    rr /= segs;
+    const int x = (int)(rr % nx);
+    const long long vi = rr / nx;
+    const long long stride = (long long)nx * ny;
+    const long long base = vi * stride * nz + (long long)x * ny + y;
+    const int z0 = sgi * kBoxSeg, z1 = min(nz, z0 + kBoxSeg);
     double acc = 0.0;
-    for (int k = 0; k <= hw && k < len; ++k) acc += in[base + k * stride];
-    for (int i = 0; i < len; ++i) {
-        out[base + i * stride] = (float)acc;
-        const int add = i + hw + 1, sub = i - hw;
-        if (add < len) acc += in[base + add * stride];
+    for (int k = max(0, z0 - hw); k <= min(nz - 1, z0 + hw); ++k) acc += in[base + k * stride];
+    for (int z = z0; z < z1; ++z) {
+        out[base + z * stride] = (float)acc;
+        const int add = z + hw + 1, sub = z - hw;
+        if (add < nz) acc += in[base + add * stride];
         if (sub >= 0) acc -= in[base + sub * stride];
     }
 }
+
 
 // out = in * scale (the atlas MRI as the tiled kernels stage it).
 __global__ void scale_copy(const float* __restrict__ in, float* __restrict__ out, long long n,
@@ -1366,26 +1358,21 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     }
     if (rc) return rc;
 
-    // 2. window sums of every atlas CT: y into tmp, x into boxL, z into tmp
+    // 2. window sums of every atlas CT: y and x into tmp, z into boxL
     float* boxL = reinterpret_cast<float*>(w + o_boxL);
     float* tmp = reinterpret_cast<float*>(w + o_tmp);
-    const long long rows_y = (long long)n * pb->nz * pb->nx;
-    const size_t ylds = sizeof(float) * 4 * ((size_t)pb->ny + 2 * hw);
-    if (ylds <= 64 * 1024) {
-        hipLaunchKernelGGL(box_y, dim3((unsigned)((rows_y + 3) / 4)), dim3(256), ylds, s,
-                           d_label, tmp, pb->ny, rows_y, hw);
-    } else {
-        const unsigned bb = (unsigned)std::min<long long>((N * n + 255) / 256, 65536);
-        hipLaunchKernelGGL(box_pass, dim3(bb), dim3(256), 0, s, d_label, tmp, pb->nx, pb->ny,
-                           pb->nz, n, hw, 2);
+    {
+        const long long planes = (long long)n * pb->nz;
+        const long long tiles = (long long)((pb->nx + kBoxTX - 1) / kBoxTX) *
+                                ((pb->ny + kBoxTY - 1) / kBoxTY);
+        const size_t blds = sizeof(float) * (kBoxTX + 2 * hw) * (2 * kBoxTY + 2 * hw);
+        hipLaunchKernelGGL(box_xy, dim3((unsigned)(tiles * planes)), dim3(256), blds, s, d_label,
+                           tmp, pb->nx, pb->ny, planes, hw);
+        const long long lz = (long long)n * pb->nx * pb->ny * ((pb->nz + kBoxSeg - 1) / kBoxSeg);
+        hipLaunchKernelGGL(box_z, dim3((unsigned)((lz + 255) / 256)), dim3(256), 0, s, tmp, boxL,
+                           pb->nx, pb->ny, pb->nz, n, hw);
+        if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
     }
-    const long long lx = (long long)n * pb->nz * pb->ny, lz = (long long)n * pb->nx * pb->ny;
-    hipLaunchKernelGGL(box_line, dim3((unsigned)((lx + 255) / 256)), dim3(256), 0, s, tmp, boxL,
-                       pb->nx, pb->ny, pb->nz, n, hw, 1);
-    hipLaunchKernelGGL(box_line, dim3((unsigned)((lz + 255) / 256)), dim3(256), 0, s, boxL, tmp,
-                       pb->nx, pb->ny, pb->nz, n, hw, 0);
-    if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
-    std::swap(boxL, tmp);
 
     // 3. selection; the lattice samples are 3x3x3 offsets at 0 and
     // +-ceil(2 hw / 3) (all 27 offsets for NH = 3)
