@@ -258,7 +258,7 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--shape", default="112,222,222", help="nz,nx,ny")
     ap.add_argument("--atlases", type=int, default=10)
     ap.add_argument("--nh", type=int, default=11)
@@ -448,7 +448,10 @@ def main() -> int:
         cpu = {"value": full / full_time, "unit": "voxels/s", "cores": threads, "kind": kind,
                "sample": f"crop of the same inputs: {planes} interior z planes x {rows} "
                          f"interior x rows x all y ({vox_s} interior voxels, n={n}, NH={nh}) "
-                         f"in {dt:.2f}s, scaled by interior voxels to the full volume"}
+                         f"in {dt:.2f}s, scaled by interior voxels to the full volume",
+               "host_logical_cpus": os.cpu_count(),
+               "parallelism": f"OpenMP over z as the reference (cpp:1374): at most "
+                              f"{interior(nz, r)} threads have work at full size"}
         log(f"cpu baseline: {cpu}")
 
     line = {
