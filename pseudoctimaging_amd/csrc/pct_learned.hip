@@ -250,7 +250,9 @@ struct EmbedWinArgs {
     int r, hw;
     int tiles_x, tiles_y, oy, wx;   // tile geometry of the selection
     int zlo, nzr;                   // z range [zlo, zlo + nzr) covering every window
-    int jobs;                       // nvol * tiles_x * tiles_y * nzr
+    const int* zlist;               // or, when set, only these nzl planes
+    int nzl;
+    int jobs;                       // nvol * tiles_x * tiles_y * (nzl or nzr)
     float mu, inv_sd;
 };
 
@@ -293,8 +295,9 @@ __global__ __launch_bounds__(kEmbWaves * 64) void embed_windows_kernel(EmbedWinA
         const bool active = job < a.jobs;
         int z = 0, tx = 0, ty = 0, vi = 0;
         if (active) {
-            z = a.zlo + job % a.nzr;
-            int t = job / a.nzr;
+            const int nzj = a.zlist ? a.nzl : a.nzr;
+            z = a.zlist ? a.zlist[job % nzj] : a.zlo + job % nzj;
+            int t = job / nzj;
             ty = t % a.tiles_y;
             t /= a.tiles_y;
             tx = t % a.tiles_x;
@@ -515,7 +518,8 @@ static int launch_embed(const float* params, const void* d_model, const float* v
 // clusters alone are read); tile geometry as in pct_fuse_pruned_dev.
 static int launch_embed_windows(const float* params, const void* d_model, const float* vol,
                                 int nvol, int whw, const pct_problem* pb, const TileGeometry& g,
-                                int oy, int wx, int bz, unsigned short* emb, hipStream_t stream) {
+                                int oy, int wx, int bz, unsigned short* emb, hipStream_t stream,
+                                const int* d_zlist = nullptr, int nzl = 0) {
     EmbedWinArgs a;
     a.vol = vol;
     a.emb = emb;
@@ -537,7 +541,9 @@ static int launch_embed_windows(const float* params, const void* d_model, const 
     a.zlo = std::max(z_first - whw, 0);
     const int zhi = std::min(z_last + 1 + whw, pb->nz - 1);
     a.nzr = zhi - a.zlo + 1;
-    const long long jobs = (long long)a.nvol * a.tiles_x * a.tiles_y * a.nzr;
+    a.zlist = d_zlist;
+    a.nzl = nzl;
+    const long long jobs = (long long)a.nvol * a.tiles_x * a.tiles_y * (d_zlist ? nzl : a.nzr);
     if (jobs >= INT_MAX) return PCT_ERR_TOO_LARGE;
     a.jobs = (int)jobs;
     a.mu = params[kOffMu];
@@ -654,6 +660,7 @@ struct SelectArgs {
     int max_slabs;                // distinct dz slabs the kept offsets may use; 0 = all
     int n_lattice_dz;
     int lattice_dz[3];            // dz of the lattice samples (always allowed)
+    unsigned score_bits;          // dz indices (dz + hw) whose offsets are scored
 };
 
 constexpr int kSelThreads = 512;
@@ -891,15 +898,39 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
             for (int j = 0; j < 4; ++j) g[(4 * lg + j) * pvs + wl] = d[j] + nn;
         }
     };
+    // Scores are computed for the dz planes in a.score_bits (every plane, or
+    // only those kept offsets may come from); a window plane is computed
+    // when one of them reads it, and the loads of the next such plane are in
+    // flight during each round of gathers.
+    auto scored = [&](int dzi) { return dzi >= 0 && dzi < nw && ((a.score_bits >> dzi) & 1u); };
+    auto plane_needed = [&](int wz) { return scored(wz) || scored(wz - 1); };
     uint2 bq[kTilesPerWave];
-    load_plane(0, bq);
-    plane_g(0, bq);
-    load_plane(1, bq);
+    int loaded = -1;   // window plane whose B operands sit in bq (uniform)
+    auto ensure = [&](int wz) {
+        if (loaded != wz) {
+            load_plane(wz, bq);
+            loaded = wz;
+        }
+    };
+    if (plane_needed(0)) {
+        ensure(0);
+        plane_g(0, bq);
+    }
+    if (plane_needed(1)) ensure(1);
     for (int dzi = 0; dzi < nw; ++dzi) {
-        plane_g((dzi + 1) & 1, bq);
-        if (dzi + 2 < WZ) load_plane(dzi + 2, bq);   // in flight during the scores
+        const bool np = plane_needed(dzi + 1), sc_now = scored(dzi);
+        if (!np && !sc_now) continue;   // uniform over the workgroup
+        if (np) {
+            ensure(dzi + 1);
+            plane_g((dzi + 1) & 1, bq);
+        }
+        for (int w2 = dzi + 2; w2 < WZ; ++w2)
+            if (plane_needed(w2)) {
+                ensure(w2);   // in flight during the gathers
+                break;
+            }
         __syncthreads();
-        if ((int)threadIdx.x < nw2) {
+        if (sc_now && (int)threadIdx.x < nw2) {
             const int dxi = threadIdx.x / nw, dyi = threadIdx.x - dxi * nw;
             const float* g0 = ring + (dzi & 1) * 16 * pvs;         // probes with z = 0
             const float* g1 = ring + ((dzi + 1) & 1) * 16 * pvs;   // probes with z = 1
@@ -925,7 +956,7 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
     {
         const int d = threadIdx.x >> 4, part = threadIdx.x & 15;
         float m = __builtin_huge_valf();
-        if (d < nw)
+        if (d < nw && scored(d))
             for (int c = d * nw2 + part; c < (d + 1) * nw2; c += 16) m = fminf(m, key[c]);
 #pragma unroll
         for (int off = 8; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off, 16));
@@ -1335,6 +1366,7 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     const size_t o_sel = off; off = align(off + sizeof(int) * (size_t)items * topk);
     const size_t o_flag = off; off = align(off + (size_t)items);
     const size_t o_scaled = off; off = align(off + sizeof(float) * (size_t)N * n);
+    const size_t o_zlist = off; off = align(off + sizeof(int) * (size_t)pb->nz);
     void* ws = nullptr;
     rc = ensure_workspace(dev, off, &ws);
     if (rc) return rc;
@@ -1342,6 +1374,57 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     const void* model = nullptr;
     rc = device_model(dev, params, s, &model);
     if (rc) return rc;
+
+    // the lattice samples are 3x3x3 offsets at 0 and +-ceil(2 hw / 3) (all
+    // 27 offsets for NH = 3)
+    const int nw = 2 * hw + 1;
+    int samples[kSamples] = {0};
+    int n_samples = 0;
+    {
+        const int q = std::max(1, (2 * hw + 2) / 3);
+        const int lat[3] = {-q, 0, q};
+        const int want = std::min(std::max(opts->samples, 0), kSamples);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int c = 0; c < 3; ++c)
+                    if (n_samples < want)
+                        samples[n_samples++] =
+                            ((lat[a] + hw) * nw + (lat[b] + hw)) * nw + (lat[c] + hw);
+        std::sort(samples, samples + n_samples);
+    }
+    int lattice_dz[3] = {0, 0, 0}, n_lattice_dz = 0;
+    for (int i = 0; i < n_samples; ++i) {
+        const int dz = samples[i] / (nw * nw) - hw;
+        bool seen = false;
+        for (int l = 0; l < n_lattice_dz; ++l) seen |= lattice_dz[l] == dz;
+        if (!seen) lattice_dz[n_lattice_dz++] = dz;
+    }
+    // With at most as many slabs as lattice planes, kept offsets come from
+    // the lattice planes only: the selection scores those dz alone, and the
+    // atlas embeddings are needed only on the window planes they read.
+    const bool lattice_only =
+        opts->max_slabs > 0 && opts->max_slabs <= n_lattice_dz && n_lattice_dz < nw;
+    unsigned score_bits = (1u << nw) - 1u;
+    if (lattice_only) {
+        score_bits = 0;
+        for (int l = 0; l < n_lattice_dz; ++l) score_bits |= 1u << (lattice_dz[l] + hw);
+    }
+    std::vector<int> zlist;
+    if (lattice_only && !opts->embed_all) {
+        // window plane wz (0 .. 2 hw + 1) of tile row tz sits at z = pz0 - hw
+        // + wz, clamped as the selection clamps it; plane wz is read by the
+        // scored dz index wz (z = 0 probes) and wz - 1 (z = 1 probes)
+        std::vector<char> need((size_t)pb->nz, 0);
+        for (int tz = 0; tz < g.tiles_z; ++tz) {
+            const int pz0 = probe_origin(tz, sh.bz, g.r, pb->nz, 2);
+            for (int wz = 0; wz <= 2 * hw + 1; ++wz) {
+                const bool rd = ((score_bits >> wz) & 1u) || (wz >= 1 && ((score_bits >> (wz - 1)) & 1u));
+                if (rd) need[(size_t)std::min(std::max(pz0 - hw + wz, 0), pb->nz - 1)] = 1;
+            }
+        }
+        for (int z = 0; z < pb->nz; ++z)
+            if (need[(size_t)z]) zlist.push_back(z);
+    }
 
     // 1. embeddings of the target and of every atlas MRI
     unsigned short* embT = reinterpret_cast<unsigned short*>(w + o_embT);
@@ -1352,9 +1435,17 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
         if (!rc) rc = launch_embed(params, model, d_atlas, pb->nx, pb->ny, pb->nz, n, embA, s);
     } else {
         rc = launch_embed_windows(params, model, d_img, 1, 0, pb, g, t_oy, t_wx, sh.bz, embT, s);
+        int* d_zlist = nullptr;
+        if (!rc && !zlist.empty()) {
+            // pageable source: the copy has taken the data when the call returns
+            d_zlist = reinterpret_cast<int*>(w + o_zlist);
+            if (hipMemcpyAsync(d_zlist, zlist.data(), zlist.size() * sizeof(int),
+                               hipMemcpyHostToDevice, s) != hipSuccess)
+                rc = PCT_ERR_HIP;
+        }
         if (!rc)
             rc = launch_embed_windows(params, model, d_atlas, n, hw, pb, g, t_oy, t_wx, sh.bz,
-                                      embA, s);
+                                      embA, s, d_zlist, (int)zlist.size());
     }
     if (rc) return rc;
 
@@ -1374,23 +1465,7 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
         if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
     }
 
-    // 3. selection; the lattice samples are 3x3x3 offsets at 0 and
-    // +-ceil(2 hw / 3) (all 27 offsets for NH = 3)
-    const int nw = 2 * hw + 1;
-    int samples[kSamples] = {0};
-    int n_samples = 0;
-    {
-        const int q = std::max(1, (2 * hw + 2) / 3);
-        const int lat[3] = {-q, 0, q};
-        const int want = std::min(std::max(opts->samples, 0), kSamples);
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b)
-                for (int c = 0; c < 3; ++c)
-                    if (n_samples < want)
-                        samples[n_samples++] =
-                            ((lat[a] + hw) * nw + (lat[b] + hw)) * nw + (lat[c] + hw);
-        std::sort(samples, samples + n_samples);
-    }
+    // 3. selection
     SelectArgs sa;
     sa.embT = embT;
     sa.embA = embA;
@@ -1411,13 +1486,9 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     sa.topk = topk;
     sa.margin = opts->margin;
     sa.max_slabs = opts->max_slabs;
-    sa.n_lattice_dz = 0;
-    for (int i = 0; i < n_samples; ++i) {
-        const int dz = samples[i] / (nw * nw) - hw;
-        bool seen = false;
-        for (int l = 0; l < sa.n_lattice_dz; ++l) seen |= sa.lattice_dz[l] == dz;
-        if (!seen) sa.lattice_dz[sa.n_lattice_dz++] = dz;
-    }
+    sa.n_lattice_dz = n_lattice_dz;
+    for (int l = 0; l < 3; ++l) sa.lattice_dz[l] = lattice_dz[l];
+    sa.score_bits = score_bits;
     if (ncand > kSelThreads * kSelPerThread) return PCT_ERR_UNSUPPORTED;
     // largest compacted candidate count: every slab, or the allowed ones
     const int slabs = (opts->max_slabs <= 0 || opts->max_slabs >= nw)
