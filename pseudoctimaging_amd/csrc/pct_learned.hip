@@ -10,8 +10,8 @@
 //                 16x16 output tile of one layer is, lane for lane, the B
 //                 operand of the next layer, so no data moves between layers;
 //                 the 28 weight fragments stay in VGPRs.
-//  box_xy, box_z  NH-wide box sums of the atlas CT (an LDS pass over y and x,
-//                 running sums over z).
+//  box_xyz        NH-wide box sums of the atlas CT in one pass (per plane an
+//                 LDS pass over y and x, then running sums over z).
 //  select_kernel  per (tile, atlas): embedding distance of every window
 //                 offset summed over a 4x2x2 probe cluster at the tile centre
 //                 (probe-by-voxel dot products on BF16 MFMA, one window
@@ -559,80 +559,149 @@ static int launch_embed_windows(const float* params, const void* d_model, const 
 // window box sums of the atlas CT
 // ---------------------------------------------------------------------------
 
-// The sums along y and x in one pass: a block stages a (16 + 2 hw) x
-// (64 + 2 hw) tile of one plane in LDS (zeros outside the volume), sums
-// each row over y, then the row sums over x.
-constexpr int kBoxTX = 16, kBoxTY = 64;
+// The NH^3 window sums in one pass.  A block owns a kBoxTX x kBoxTY column
+// tile of one volume over a run of kBoxRun output planes and sweeps the
+// planes that feed it.  Each plane's tile, with a halo of HW (zeros outside
+// the volume), goes through LDS and is summed over y (4 outputs per task,
+// from 128-bit LDS reads), then over x.  The plane sums enter a running sum
+// over z in double, and the last 2 HW + 1 of them stay in registers for the
+// subtraction; the double sum of float plane sums is exact in practice, so
+// the result does not depend on where a run starts.  The next plane's loads
+// are in flight during the sums.  Every thread owns 4 consecutive x of one
+// y of the tile.
+constexpr int kBoxTX = 16, kBoxTY = 64, kBoxRun = 32;
 
-__global__ __launch_bounds__(256) void box_xy(const float* __restrict__ in,
-                                              float* __restrict__ out, int nx, int ny,
-                                              long long planes, int hw) {
-    extern __shared__ float sh[];
-    const int rows = kBoxTX + 2 * hw, cols = kBoxTY + 2 * hw;
-    float* T = sh;                  // [rows][cols]
-    float* Y = sh + rows * cols;    // [rows][kBoxTY] sums over y
+template <int HW>
+__global__ __launch_bounds__(256) void box_xyz(const float* __restrict__ in,
+                                               float* __restrict__ out, int nx, int ny, int nz,
+                                               int nvol) {
+    constexpr int K = 2 * HW + 1;
+    constexpr int R = kBoxTX + 2 * HW, C = kBoxTY + 2 * HW;
+    constexpr int NQ = (4 + 2 * HW + 3) / 4;     // float4 reads per y-sum task
+    constexpr int CP = kBoxTY - 4 + 4 * NQ;      // LDS row pitch (>= C, multiple of 4)
+    constexpr int NLD = (R * C + 255) / 256;     // tile loads per thread
+    constexpr int NYT = R * (kBoxTY / 4);        // y-sum tasks
+    static_assert(CP >= C && CP % 4 == 0, "row pitch");
+    __shared__ __align__(16) float T[R * CP];
+    __shared__ __align__(16) float Y[R * kBoxTY];
+
     const int tiles_x = (nx + kBoxTX - 1) / kBoxTX, tiles_y = (ny + kBoxTY - 1) / kBoxTY;
-    const long long b = blockIdx.x;
+    const int runs = (nz + kBoxRun - 1) / kBoxRun;
+    long long b = blockIdx.x;
     const int ty = (int)(b % tiles_y);
-    const int tx = (int)((b / tiles_y) % tiles_x);
-    const long long pl = b / ((long long)tiles_y * tiles_x);
-    if (pl >= planes) return;   // uniform over the block
+    b /= tiles_y;
+    const int tx = (int)(b % tiles_x);
+    b /= tiles_x;
+    const int run = (int)(b % runs);
+    const long long vi = b / runs;
+    if (vi >= nvol) return;   // uniform over the block
     const int x0 = tx * kBoxTX, y0 = ty * kBoxTY;
-    const float* __restrict__ src = in + pl * nx * (long long)ny;
-    for (int i = threadIdx.x; i < rows * cols; i += 256) {
-        const int r = i / cols, c = i - (i / cols) * cols;
-        const int x = x0 - hw + r, y = y0 - hw + c;
-        T[i] = (x >= 0 && x < nx && y >= 0 && y < ny) ? src[(long long)x * ny + y] : 0.f;
+    const int z0 = run * kBoxRun, z1 = min(nz, z0 + kBoxRun);
+    const long long plane = (long long)nx * ny;
+    const float* __restrict__ src = in + vi * plane * nz;
+    float* __restrict__ dst = out + vi * plane * nz;
+
+    // load slots: in-plane offset of the tile element, -1 outside the
+    // volume (a zero), -2 past the tile
+    int lofs[NLD];
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+        const int e = (int)threadIdx.x + i * 256;
+        const int rr = e / C, cc = e - (e / C) * C;
+        const int x = x0 - HW + rr, y = y0 - HW + cc;
+        lofs[i] = e >= R * C ? -2 : (x >= 0 && x < nx && y >= 0 && y < ny) ? x * ny + y : -1;
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < rows * kBoxTY; i += 256) {
-        const int r = i / kBoxTY, c = i - (i / kBoxTY) * kBoxTY;
-        float acc = 0.f;
-        for (int k = 0; k <= 2 * hw; ++k) acc += T[r * cols + c + k];
-        Y[i] = acc;
-    }
-    __syncthreads();
-    float* __restrict__ dst = out + pl * nx * (long long)ny;
-    for (int i = threadIdx.x; i < kBoxTX * kBoxTY; i += 256) {
-        const int r = i / kBoxTY, c = i - (i / kBoxTY) * kBoxTY;
-        const int x = x0 + r, y = y0 + c;
-        if (x >= nx || y >= ny) continue;
-        float acc = 0.f;
-        for (int k = 0; k <= 2 * hw; ++k) acc += Y[(r + k) * kBoxTY + c];
-        dst[(long long)x * ny + y] = acc;
+    float v[NLD];
+    auto load = [&](int p) {
+        const float* __restrict__ sp = src + p * plane;
+#pragma unroll
+        for (int i = 0; i < NLD; ++i) v[i] = lofs[i] >= 0 ? sp[lofs[i]] : 0.f;
+    };
+
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ox = 4 * wv, oy = lane;   // outputs (x0 + ox + m, y0 + oy), m < 4
+    float ring[K][4];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int pa = max(0, z0 - HW), pe = z1 - 1 + HW;   // planes >= nz are zero
+    load(pa);
+    for (int pb = pa; pb <= pe; pb += K) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int p = pb + j;
+            if (p > pe) break;   // uniform
+            float bs[4] = {0.f, 0.f, 0.f, 0.f};
+            if (p < nz) {        // uniform
+                __syncthreads();   // the previous plane's y sums are done with T
+#pragma unroll
+                for (int i = 0; i < NLD; ++i) {
+                    const int e = (int)threadIdx.x + i * 256;
+                    if (lofs[i] != -2) T[(e / C) * CP + e - (e / C) * C] = v[i];
+                }
+                if (p + 1 < nz && p + 1 <= pe) load(p + 1);   // in flight during the sums
+                __syncthreads();
+#pragma unroll
+                for (int t0 = 0; t0 < NYT; t0 += 256) {
+                    const int t = t0 + (int)threadIdx.x;
+                    if (t < NYT) {
+                        const int rr = t >> 4, c0 = 4 * (t & 15);
+                        float wv4[4 * NQ];
+#pragma unroll
+                        for (int q = 0; q < NQ; ++q) {
+                            const float4 f = *reinterpret_cast<const float4*>(&T[rr * CP + c0 + 4 * q]);
+                            wv4[4 * q] = f.x;
+                            wv4[4 * q + 1] = f.y;
+                            wv4[4 * q + 2] = f.z;
+                            wv4[4 * q + 3] = f.w;
+                        }
+                        float4 o;
+                        float* op = reinterpret_cast<float*>(&o);
+#pragma unroll
+                        for (int m = 0; m < 4; ++m) {
+                            float sacc = 0.f;
+#pragma unroll
+                            for (int k = 0; k < K; ++k) sacc += wv4[m + k];
+                            op[m] = sacc;
+                        }
+                        *reinterpret_cast<float4*>(&Y[rr * kBoxTY + c0]) = o;
+                    }
+                }
+                __syncthreads();
+                float col[4 + 2 * HW];
+#pragma unroll
+                for (int k = 0; k < 4 + 2 * HW; ++k) col[k] = Y[(ox + k) * kBoxTY + oy];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    float sacc = 0.f;
+#pragma unroll
+                    for (int k = 0; k < K; ++k) sacc += col[m + k];
+                    bs[m] = sacc;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                acc[m] += (double)bs[m];
+                ring[j][m] = bs[m];
+            }
+            const int z = p - HW;   // the output whose window ends at p
+            if (z >= z0) {
+                const int y = y0 + oy;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int x = x0 + ox + m;
+                    if (x < nx && y < ny) dst[z * plane + (long long)x * ny + y] = (float)acc[m];
+                    // plane z - HW = p - 2 HW leaves the window; it sits in
+                    // ring slot (j - 2 HW) mod K = (j + 1) mod K
+                    if (z - HW >= pa) acc[m] -= (double)ring[(j + 1) % K][m];
+                }
+            }
+        }
     }
 }
 
-// The sums along z as running sums in double over segments of kBoxSeg
-// planes: one thread per (line, segment), the lanes of a wave on
-// consecutive y.  A segment starts from the direct sum of its first window.
-constexpr int kBoxSeg = 16;
-
-__global__ void box_z(const float* __restrict__ in, float* __restrict__ out, int nx, int ny,
-                      int nz, int nvol, int hw) {
-    const int segs = (nz + kBoxSeg - 1) / kBoxSeg;
-    const long long work = (long long)nvol * nx * ny * segs;
-    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= work) return;
-    const int y = (int)(t % ny);
-    long long rr = t / ny;
-    const int sgi = (int)(rr % segs);
-    rr /= segs;
-    const int x = (int)(rr % nx);
-    const long long vi = rr / nx;
-    const long long stride = (long long)nx * ny;
-    const long long base = vi * stride * nz + (long long)x * ny + y;
-    const int z0 = sgi * kBoxSeg, z1 = min(nz, z0 + kBoxSeg);
-    double acc = 0.0;
-    for (int k = max(0, z0 - hw); k <= min(nz - 1, z0 + hw); ++k) acc += in[base + k * stride];
-    for (int z = z0; z < z1; ++z) {
-        out[base + z * stride] = (float)acc;
-        const int add = z + hw + 1, sub = z - hw;
-        if (add < nz) acc += in[base + add * stride];
-        if (sub >= 0) acc -= in[base + sub * stride];
-    }
-}
-
+using BoxFn = void (*)(const float*, float*, int, int, int, int);
+static const BoxFn kBoxXyz[kMaxTiledHalfWindow] = {&box_xyz<1>, &box_xyz<2>, &box_xyz<3>,
+                                                   &box_xyz<4>, &box_xyz<5>, &box_xyz<6>,
+                                                   &box_xyz<7>};
 
 // out = in * scale (the atlas MRI as the tiled kernels stage it).
 __global__ void scale_copy(const float* __restrict__ in, float* __restrict__ out, long long n,
@@ -1362,7 +1431,6 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     const size_t o_embT = off; off = align(off + 32 * (size_t)N);
     const size_t o_embA = off; off = align(off + 32 * (size_t)N * n);
     const size_t o_boxL = off; off = align(off + sizeof(float) * (size_t)N * n);
-    const size_t o_tmp = off; off = align(off + sizeof(float) * (size_t)N * n);
     const size_t o_sel = off; off = align(off + sizeof(int) * (size_t)items * topk);
     const size_t o_flag = off; off = align(off + (size_t)items);
     const size_t o_scaled = off; off = align(off + sizeof(float) * (size_t)N * n);
@@ -1449,19 +1517,14 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     }
     if (rc) return rc;
 
-    // 2. window sums of every atlas CT: y and x into tmp, z into boxL
+    // 2. window sums of every atlas CT
     float* boxL = reinterpret_cast<float*>(w + o_boxL);
-    float* tmp = reinterpret_cast<float*>(w + o_tmp);
     {
-        const long long planes = (long long)n * pb->nz;
-        const long long tiles = (long long)((pb->nx + kBoxTX - 1) / kBoxTX) *
-                                ((pb->ny + kBoxTY - 1) / kBoxTY);
-        const size_t blds = sizeof(float) * (kBoxTX + 2 * hw) * (2 * kBoxTY + 2 * hw);
-        hipLaunchKernelGGL(box_xy, dim3((unsigned)(tiles * planes)), dim3(256), blds, s, d_label,
-                           tmp, pb->nx, pb->ny, planes, hw);
-        const long long lz = (long long)n * pb->nx * pb->ny * ((pb->nz + kBoxSeg - 1) / kBoxSeg);
-        hipLaunchKernelGGL(box_z, dim3((unsigned)((lz + 255) / 256)), dim3(256), 0, s, tmp, boxL,
-                           pb->nx, pb->ny, pb->nz, n, hw);
+        const long long blocks = (long long)n * ((pb->nz + kBoxRun - 1) / kBoxRun) *
+                                 ((pb->nx + kBoxTX - 1) / kBoxTX) *
+                                 ((pb->ny + kBoxTY - 1) / kBoxTY);
+        hipLaunchKernelGGL(kBoxXyz[hw - 1], dim3((unsigned)blocks), dim3(256), 0, s, d_label, boxL,
+                           pb->nx, pb->ny, pb->nz, n);
         if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
     }
 

@@ -145,6 +145,53 @@ def test_pruned_all_candidates_is_bit_exact(gpu, shape_id):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+@pytest.mark.parametrize("shape,nh", [((40, 37, 70), 11), ((30, 45, 131), 7), ((36, 33, 29), 15)])
+def test_tail_term_constant_mri_is_window_mean(gpu, shape, nh):
+    # With a constant MRI every candidate weight is exp(0) = 1, so the pruned
+    # fusion, kept offsets and lattice samples weighed exactly and every other
+    # candidate through the tail term, must return the mean of the atlas CT
+    # over the NH^3 window (the closed form of the exact fusion).  This pins
+    # the window sums of the CT on ragged tile edges.
+    import torch
+    nz, nx, ny = shape
+    n = 2
+    hw = (nh - 1) // 2
+    r = hw + 1
+    rng = np.random.default_rng(nh)
+    img = np.full(shape, 0.5, np.float32)
+    amri = np.full((n,) + shape, 0.5, np.float32)
+    lab = (rng.random((n,) + shape) * 255.0).astype(np.float32)
+    p = learned.mlp_init(2, 0.5, 1.0)
+    out = torch.full(shape, -7.0, device="cuda:0")
+    print(f"case tail window mean {shape} NH={nh}", flush=True)
+    frac = learned.fuse_pruned_device(_dev(img), _dev(amri), _dev(lab), out, nh, 2.5, 1.0, p,
+                                      topk=16, samples=27, margin=0.0)
+    torch.cuda.synchronize()
+    assert frac == 0.0
+    got = out.cpu().numpy()
+    # window sums by a 3-d summed-volume table in float64
+    c = np.zeros((n, nz + 1, nx + 1, ny + 1))
+    c[:, 1:, 1:, 1:] = lab.astype(np.float64).cumsum(1).cumsum(2).cumsum(3)
+    k = 2 * hw + 1
+    z0, z1 = r - hw, nz - r - hw   # window starts of the interior voxels
+    x0, x1 = r - hw, nx - r - hw
+    y0, y1 = r - hw, ny - r - hw
+    box = np.zeros((n, nz - 2 * r, nx - 2 * r, ny - 2 * r))
+    for dz, sz in ((k, 1), (0, -1)):
+        for dx, sx in ((k, 1), (0, -1)):
+            for dy, sy in ((k, 1), (0, -1)):
+                box += sz * sx * sy * c[:, z0 + dz:z1 + dz, x0 + dx:x1 + dx, y0 + dy:y1 + dy]
+    want = box.sum(0) / (n * k ** 3)
+    inner = got[r:nz - r, r:nx - r, r:ny - r].astype(np.float64)
+    assert np.isfinite(inner).all()
+    err = np.abs(inner - want) / np.maximum(np.abs(want), 1.0)
+    assert err.max() < 2e-5, (err.max(), np.unravel_index(err.argmax(), err.shape))
+    # the border is zero
+    border = got.copy()
+    border[r:nz - r, r:nx - r, r:ny - r] = 0.0
+    assert not border.any()
+
+
 def test_pruned_full_fallback_is_bit_exact(gpu):
     # negative margin: every item takes the full search
     import torch
