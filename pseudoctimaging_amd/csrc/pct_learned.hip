@@ -564,24 +564,27 @@ static int launch_embed_windows(const float* params, const void* d_model, const 
 // planes that feed it.  Each plane's tile, with a halo of HW (zeros outside
 // the volume), goes through LDS and is summed over y (4 outputs per task,
 // from 128-bit LDS reads), then over x.  The plane sums enter a running sum
-// over z in double, and the last 2 HW + 1 of them stay in registers for the
+// over z in double, and the last 2 HW + 2 of them stay in registers for the
 // subtraction; the double sum of float plane sums is exact in practice, so
-// the result does not depend on where a run starts.  The next plane's loads
-// are in flight during the sums.  Every thread owns 4 consecutive x of one
-// y of the tile.
-constexpr int kBoxTX = 16, kBoxTY = 64, kBoxRun = 32;
+// the result does not depend on where a run starts.  Every thread owns 2
+// consecutive x of one y of the tile, and the loads of the next two planes
+// are in flight during a plane's sums (two register buffers).
+constexpr int kBoxTX = 16, kBoxTY = 64, kBoxRun = 32, kBoxThreads = 512;
 
 template <int HW>
-__global__ __launch_bounds__(256) void box_xyz(const float* __restrict__ in,
-                                               float* __restrict__ out, int nx, int ny, int nz,
-                                               int nvol) {
+__global__ __launch_bounds__(kBoxThreads) void box_xyz(const float* __restrict__ in,
+                                                       float* __restrict__ out, int nx, int ny,
+                                                       int nz, int nvol) {
     constexpr int K = 2 * HW + 1;
+    constexpr int KR = K + 1;                    // ring length, even
     constexpr int R = kBoxTX + 2 * HW, C = kBoxTY + 2 * HW;
     constexpr int NQ = (4 + 2 * HW + 3) / 4;     // float4 reads per y-sum task
     constexpr int CP = kBoxTY - 4 + 4 * NQ;      // LDS row pitch (>= C, multiple of 4)
-    constexpr int NLD = (R * C + 255) / 256;     // tile loads per thread
+    constexpr int NLD = (R * C + kBoxThreads - 1) / kBoxThreads;   // tile loads per thread
     constexpr int NYT = R * (kBoxTY / 4);        // y-sum tasks
+    constexpr int OX = kBoxTX / (kBoxThreads / 64);   // x outputs per thread
     static_assert(CP >= C && CP % 4 == 0, "row pitch");
+    static_assert(NYT <= kBoxThreads && OX == 2, "one y-sum task and 2 outputs per thread");
     __shared__ __align__(16) float T[R * CP];
     __shared__ __align__(16) float Y[R * kBoxTY];
 
@@ -606,71 +609,69 @@ __global__ __launch_bounds__(256) void box_xyz(const float* __restrict__ in,
     int lofs[NLD];
 #pragma unroll
     for (int i = 0; i < NLD; ++i) {
-        const int e = (int)threadIdx.x + i * 256;
+        const int e = (int)threadIdx.x + i * kBoxThreads;
         const int rr = e / C, cc = e - (e / C) * C;
         const int x = x0 - HW + rr, y = y0 - HW + cc;
         lofs[i] = e >= R * C ? -2 : (x >= 0 && x < nx && y >= 0 && y < ny) ? x * ny + y : -1;
     }
-    float v[NLD];
-    auto load = [&](int p) {
+    float va[NLD], vb[NLD];
+    auto load = [&](int p, float (&v)[NLD]) {
         const float* __restrict__ sp = src + p * plane;
 #pragma unroll
         for (int i = 0; i < NLD; ++i) v[i] = lofs[i] >= 0 ? sp[lofs[i]] : 0.f;
     };
 
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int ox = 4 * wv, oy = lane;   // outputs (x0 + ox + m, y0 + oy), m < 4
-    float ring[K][4];
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int ox = OX * wv, oy = lane;   // outputs (x0 + ox + m, y0 + oy), m < OX
+    float ring[KR][OX];
+    double acc[OX] = {0.0, 0.0};
     const int pa = max(0, z0 - HW), pe = z1 - 1 + HW;   // planes >= nz are zero
-    load(pa);
-    for (int pb = pa; pb <= pe; pb += K) {
+    load(pa, va);
+    if (pa + 1 < nz) load(pa + 1, vb);
+    for (int pb = pa; pb <= pe; pb += KR) {
 #pragma unroll
-        for (int j = 0; j < K; ++j) {
+        for (int j = 0; j < KR; ++j) {
             const int p = pb + j;
             if (p > pe) break;   // uniform
-            float bs[4] = {0.f, 0.f, 0.f, 0.f};
+            float(&cur)[NLD] = (j & 1) ? vb : va;   // holds plane p
+            float bs[OX] = {0.f, 0.f};
             if (p < nz) {        // uniform
                 __syncthreads();   // the previous plane's y sums are done with T
 #pragma unroll
                 for (int i = 0; i < NLD; ++i) {
-                    const int e = (int)threadIdx.x + i * 256;
-                    if (lofs[i] != -2) T[(e / C) * CP + e - (e / C) * C] = v[i];
+                    const int e = (int)threadIdx.x + i * kBoxThreads;
+                    if (lofs[i] != -2) T[(e / C) * CP + e - (e / C) * C] = cur[i];
                 }
-                if (p + 1 < nz && p + 1 <= pe) load(p + 1);   // in flight during the sums
+                if (p + 2 < nz && p + 2 <= pe) load(p + 2, cur);   // in flight for two planes
                 __syncthreads();
+                if ((int)threadIdx.x < NYT) {
+                    const int rr = threadIdx.x >> 4, c0 = 4 * (threadIdx.x & 15);
+                    float w4[4 * NQ];
 #pragma unroll
-                for (int t0 = 0; t0 < NYT; t0 += 256) {
-                    const int t = t0 + (int)threadIdx.x;
-                    if (t < NYT) {
-                        const int rr = t >> 4, c0 = 4 * (t & 15);
-                        float wv4[4 * NQ];
-#pragma unroll
-                        for (int q = 0; q < NQ; ++q) {
-                            const float4 f = *reinterpret_cast<const float4*>(&T[rr * CP + c0 + 4 * q]);
-                            wv4[4 * q] = f.x;
-                            wv4[4 * q + 1] = f.y;
-                            wv4[4 * q + 2] = f.z;
-                            wv4[4 * q + 3] = f.w;
-                        }
-                        float4 o;
-                        float* op = reinterpret_cast<float*>(&o);
-#pragma unroll
-                        for (int m = 0; m < 4; ++m) {
-                            float sacc = 0.f;
-#pragma unroll
-                            for (int k = 0; k < K; ++k) sacc += wv4[m + k];
-                            op[m] = sacc;
-                        }
-                        *reinterpret_cast<float4*>(&Y[rr * kBoxTY + c0]) = o;
+                    for (int q = 0; q < NQ; ++q) {
+                        const float4 f = *reinterpret_cast<const float4*>(&T[rr * CP + c0 + 4 * q]);
+                        w4[4 * q] = f.x;
+                        w4[4 * q + 1] = f.y;
+                        w4[4 * q + 2] = f.z;
+                        w4[4 * q + 3] = f.w;
                     }
+                    float4 o;
+                    float* op = reinterpret_cast<float*>(&o);
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        float sacc = 0.f;
+#pragma unroll
+                        for (int k = 0; k < K; ++k) sacc += w4[m + k];
+                        op[m] = sacc;
+                    }
+                    *reinterpret_cast<float4*>(&Y[rr * kBoxTY + c0]) = o;
                 }
                 __syncthreads();
-                float col[4 + 2 * HW];
+                float col[OX + 2 * HW];
 #pragma unroll
-                for (int k = 0; k < 4 + 2 * HW; ++k) col[k] = Y[(ox + k) * kBoxTY + oy];
+                for (int k = 0; k < OX + 2 * HW; ++k) col[k] = Y[(ox + k) * kBoxTY + oy];
 #pragma unroll
-                for (int m = 0; m < 4; ++m) {
+                for (int m = 0; m < OX; ++m) {
                     float sacc = 0.f;
 #pragma unroll
                     for (int k = 0; k < K; ++k) sacc += col[m + k];
@@ -678,7 +679,7 @@ __global__ __launch_bounds__(256) void box_xyz(const float* __restrict__ in,
                 }
             }
 #pragma unroll
-            for (int m = 0; m < 4; ++m) {
+            for (int m = 0; m < OX; ++m) {
                 acc[m] += (double)bs[m];
                 ring[j][m] = bs[m];
             }
@@ -686,12 +687,12 @@ __global__ __launch_bounds__(256) void box_xyz(const float* __restrict__ in,
             if (z >= z0) {
                 const int y = y0 + oy;
 #pragma unroll
-                for (int m = 0; m < 4; ++m) {
+                for (int m = 0; m < OX; ++m) {
                     const int x = x0 + ox + m;
                     if (x < nx && y < ny) dst[z * plane + (long long)x * ny + y] = (float)acc[m];
                     // plane z - HW = p - 2 HW leaves the window; it sits in
-                    // ring slot (j - 2 HW) mod K = (j + 1) mod K
-                    if (z - HW >= pa) acc[m] -= (double)ring[(j + 1) % K][m];
+                    // ring slot (j - 2 HW) mod KR = (j + 2) mod KR
+                    if (z - HW >= pa) acc[m] -= (double)ring[(j + 2) % KR][m];
                 }
             }
         }
@@ -1523,7 +1524,7 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
         const long long blocks = (long long)n * ((pb->nz + kBoxRun - 1) / kBoxRun) *
                                  ((pb->nx + kBoxTX - 1) / kBoxTX) *
                                  ((pb->ny + kBoxTY - 1) / kBoxTY);
-        hipLaunchKernelGGL(kBoxXyz[hw - 1], dim3((unsigned)blocks), dim3(256), 0, s, d_label, boxL,
+        hipLaunchKernelGGL(kBoxXyz[hw - 1], dim3((unsigned)blocks), dim3(kBoxThreads), 0, s, d_label, boxL,
                            pb->nx, pb->ny, pb->nz, n);
         if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
     }
