@@ -929,22 +929,27 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
     // B operands of one plane: wave wv takes tiles wv, wv + 8, ...; lane
     // holds dims 4 lg .. 4 lg + 3 of voxel 16 tile + ln
     constexpr int kTilesPerWave = 3;   // 18 tiles at NH = 15
-    auto load_plane = [&](int wz, uint2 (&b)[kTilesPerWave]) {
-        const int zz = min(max(pz0 - hw + wz, 0), a.nz - 1);
+    // in-plane element offset of the lane's window voxel per tile (the same
+    // on every plane), -1 past the window
+    long long xyo[kTilesPerWave];
 #pragma unroll
-        for (int i = 0; i < kTilesPerWave; ++i) {
-            const int tile = wv + i * (kSelThreads / 64);
-            const int wl = tile * 16 + ln;
-            uint2 v = make_uint2(0u, 0u);
-            if (tile < ntile && wl < pv) {
-                const int wy = wl % WY, wx = wl / WY;
-                const int yy = min(max(py0 - hw + wy, 0), a.ny - 1);
-                const int xx = min(max(px0 - hw + wx, 0), a.nx - 1);
-                v = *reinterpret_cast<const uint2*>(
-                    eA + (zz * plane + (long long)xx * a.ny + yy) * 16 + 4 * lg);
-            }
-            b[i] = v;
+    for (int i = 0; i < kTilesPerWave; ++i) {
+        const int tile = wv + i * (kSelThreads / 64);
+        const int wl = tile * 16 + ln;
+        xyo[i] = -1;
+        if (tile < ntile && wl < pv) {
+            const int wy = wl % WY, wx = wl / WY;
+            const int yy = min(max(py0 - hw + wy, 0), a.ny - 1);
+            const int xx = min(max(px0 - hw + wx, 0), a.nx - 1);
+            xyo[i] = ((long long)xx * a.ny + yy) * 16 + 4 * lg;
         }
+    }
+    auto load_plane = [&](int wz, uint2 (&b)[kTilesPerWave]) {
+        const long long zo = (long long)min(max(pz0 - hw + wz, 0), a.nz - 1) * plane * 16;
+#pragma unroll
+        for (int i = 0; i < kTilesPerWave; ++i)
+            b[i] = xyo[i] >= 0 ? *reinterpret_cast<const uint2*>(eA + zo + xyo[i])
+                               : make_uint2(0u, 0u);
     };
     auto plane_g = [&](int slot, const uint2 (&b)[kTilesPerWave]) {
         float* g = ring + slot * 16 * pvs;
