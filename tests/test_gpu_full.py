@@ -82,6 +82,54 @@ def test_cfg3_atlas_order_invariance(gpu):
     np.testing.assert_allclose(b[inner], a[inner], rtol=2e-6)
 
 
+def test_cfg3_linear_in_ct(gpu):
+    # The weights depend on the MRI only, so out(a L + b) = a out(L) + b
+    # holds for the whole 222x222x112, n=10, NH=11 volume.
+    g = load("cfg3_sample.npz")
+    nx, ny, nz, n, nh = (int(v) for v in g["params"][:5])
+    img, amri, act = pct.synth_tissue(nx, ny, nz, n, 3, False, int(g["params"][7]))
+    lab = act.copy()
+    pct.normalize_labels(lab)
+    a = pct.multi_patch_based_segmentation_gpu(img, amri, lab, nh, 2.5, 1.0)
+    alpha, beta = 0.5, 12.0
+    lab2 = (alpha * lab + beta).astype(np.float32)
+    b = pct.multi_patch_based_segmentation_gpu(img, amri, lab2, nh, 2.5, 1.0)
+    r = (nh - 1) // 2 + 1
+    inner = (slice(r, nz - r), slice(r, nx - r), slice(r, ny - r))
+    want = alpha * a[inner].astype(np.float64) + beta
+    # 1e-3 in label units is 0.016 HU
+    assert np.abs(b[inner] - want).max() < 1e-3
+    assert not b[0].any() and not b[:, :, ny - 1].any()
+
+
+def test_cfg3_constant_mri_is_window_mean(gpu):
+    # A constant MRI makes every weight exp(0) = 1: the fusion is the mean of
+    # the atlas CT over the NH^3 window, at the full 222x222x112, n=10 size.
+    g = load("cfg3_sample.npz")
+    nx, ny, nz, n, nh = (int(v) for v in g["params"][:5])
+    _, _, act = pct.synth_tissue(nx, ny, nz, n, 3, False, int(g["params"][7]))
+    lab = act.copy()
+    pct.normalize_labels(lab)
+    img = np.full((nz, nx, ny), 0.5, np.float32)
+    amri = np.full((n, nz, nx, ny), 0.5, np.float32)
+    got = pct.multi_patch_based_segmentation_gpu(img, amri, lab, nh, 2.5, 1.0)
+    hw = (nh - 1) // 2
+    r = hw + 1
+    k = 2 * hw + 1
+    c = np.zeros((nz + 1, nx + 1, ny + 1))
+    c[1:, 1:, 1:] = lab.astype(np.float64).sum(0).cumsum(0).cumsum(1).cumsum(2)
+    s0 = [slice(r - hw, d - r - hw) for d in (nz, nx, ny)]
+    box = np.zeros((nz - 2 * r, nx - 2 * r, ny - 2 * r))
+    for mz in (0, 1):
+        for mx in (0, 1):
+            for my in (0, 1):
+                sl = tuple(slice(s.start + k * m, s.stop + k * m) for s, m in zip(s0, (mz, mx, my)))
+                box += (-1) ** (3 - mz - mx - my) * c[sl]
+    want = box / (n * k ** 3)
+    inner = got[r:nz - r, r:nx - r, r:ny - r].astype(np.float64)
+    assert np.abs(inner - want).max() < 1e-3
+
+
 def test_cfg1_full_volume(gpu):
     with open(os.path.join(GOLDEN, "cfg1.json")) as f:
         g = json.load(f)
