@@ -208,9 +208,10 @@ def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup,
 
 def end_to_end_leg(img, amri, lab, nh, sigma, beta, reps):
     """The region the reference's GPU program times around its SM call
-    (src/GPU/PatchBasedPseudoCT.cu:2152-2188): host buffers in, per-call device
-    allocation, three host-to-device copies, the fusion, the copy back and
-    the post chain (NaN fill, round, HU) on the host."""
+    (src/GPU/PatchBasedPseudoCT.cu:2152-2188): host buffers in, three
+    host-to-device copies, the fusion, the copy back and the post chain (NaN
+    fill, round, HU) on the host.  The reference allocates device memory per
+    call; pct_fuse_host keeps its device buffers between calls."""
     import numpy as np
 
     import pseudoctimaging_amd as pct
@@ -249,9 +250,11 @@ def end_to_end_leg(img, amri, lab, nh, sigma, beta, reps):
             "pseudo_ct_host_ms": one_s * 1e3, "pseudo_ct_host_same_output": same,
             "pcie_bytes": 4 * nx * ny * nz * (2 + 2 * n) + 4 * nx * ny * nz,
             "reps": reps, "statistic": "median",
-            "region": "pct_fuse_host (device alloc, 3 H2D, fusion, D2H, free) + host "
-                      "NaN fill, round and HU rescale; pseudo_ct_host_ms: the same region "
-                      "as pct_pseudo_ct_host, the post chain on the device"}
+            "region": "pct_fuse_host (3 H2D through pinned staging, overlapped with the "
+                      "fusion; D2H; device buffers cached between calls, where the reference "
+                      "allocates per call) + host NaN fill, round and HU rescale; "
+                      "pseudo_ct_host_ms: the same region as pct_pseudo_ct_host, the post "
+                      "chain on the device"}
 
 
 def main() -> int:

@@ -88,9 +88,11 @@ int pct_normalize_labels_dev(float* d_label, size_t n, pct_stream_t stream);
  * Host buffers; copies in, runs and copies `out` back like the reference
  * wrapper (cu:1687-1744), and returns when `out` is written.  The atlas
  * copies go in chunks that overlap the fusion of the chunks already copied.
- * The device buffers stay allocated for the next host-buffer call (grown on
- * demand; pct_release frees them); host-buffer calls on one device run one
- * at a time.  `out` receives every voxel: interior = fused value (NaN where
+ * Copies in both directions pass through 64 MiB of pinned host memory
+ * (four 16 MiB buffers, filled and emptied by the OpenMP threads).  The
+ * device buffers and the pinned buffers stay allocated for the next
+ * host-buffer call (grown on demand; pct_release frees them); host-buffer
+ * calls on one device run one at a time.  `out` receives every voxel: interior = fused value (NaN where
  * |P| or |S| <= DBL_EPSILON, cpp:1437-1443), border = 0 (cu:1693).
  * `choice` is a pct_exec_choice (GM, GM2 or SM).  The output equals
  * pct_fuse_dev's bit for bit. */

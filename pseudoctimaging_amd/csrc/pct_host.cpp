@@ -1,6 +1,7 @@
 // Host-side part of libpct.so: pre/post-processing around the fusion hot
 // path, the MatrixIO text format, the configuration file of the reference
 // program, and seeded input generators.  None of this computes the fusion.
+#include <omp.h>
 #include <sys/stat.h>
 
 #include <algorithm>
@@ -25,6 +26,23 @@ uint64_t splitmix64(uint64_t& state) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
     return z ^ (z >> 31);
+}
+
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+    constexpr size_t kSerial = (size_t)4 << 20;
+    if (bytes < kSerial) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const int nt = std::max(1, omp_get_max_threads());
+    const size_t part = ((bytes + nt - 1) / nt + 4095) & ~(size_t)4095;
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < nt; ++t) {
+        const size_t o = (size_t)t * part;
+        if (o < bytes)
+            std::memcpy(static_cast<char*>(dst) + o, static_cast<const char*>(src) + o,
+                        std::min(part, bytes - o));
+    }
 }
 
 namespace {

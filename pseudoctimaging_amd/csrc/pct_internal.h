@@ -35,4 +35,8 @@ int validate_problem(const pct_problem* pb);
 // Host-side helpers implemented in pct_host.cpp.
 uint64_t splitmix64(uint64_t& state);
 
+// memcpy split over the OpenMP threads in page-aligned parts (a single
+// memcpy below 4 MiB).
+void parallel_copy(void* dst, const void* src, size_t bytes);
+
 }  // namespace pct

@@ -251,6 +251,31 @@ def test_host_call_overlapped_copies_equal_device_call(gpu, case):
         pct.force_tile_config(-1, 0)
 
 
+@pytest.mark.parametrize("choice", [pct.CHOICE_SM, pct.CHOICE_GM2])
+def test_host_call_staging_of_large_buffers(gpu, choice):
+    """Host buffers larger than the pinned staging ring (four 16 MiB pieces):
+    75 MB volumes go over in five pieces each, in and out, and the output
+    equals the device-buffer call bit for bit."""
+    import torch
+    nx, ny, nz, n, nh = 384, 384, 128, 2, 3
+    print(f"case host staging {nx}x{ny}x{nz} n={n} NH={nh} choice={choice}", flush=True)
+    rng = np.random.default_rng(17)
+    img = rng.random((nz, nx, ny), dtype=np.float32)
+    amri = rng.random((n, nz, nx, ny), dtype=np.float32)
+    lab = (rng.random((n, nz, nx, ny), dtype=np.float32) * 255.0).astype(np.float32)
+    host = pct.multi_patch_based_segmentation_gpu(img, amri, lab, nh, 2.5, 1.0, choice)
+    dev = torch.device("cuda:0")
+    out = torch.empty(img.shape, device=dev)
+    pct.fuse_device(torch.from_numpy(img).to(dev), torch.from_numpy(amri).to(dev),
+                    torch.from_numpy(lab).to(dev), out, nh, 2.5, 1.0, choice)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), host.view(np.uint32))
+    # the post chain on the device, through the same staging
+    hu = pct.pseudo_ct_hu(img, amri, lab, nh, 2.5, 1.0, choice)
+    ref = pct.postprocess(host.copy(), nh)
+    assert np.array_equal(hu.view(np.uint32), ref.view(np.uint32))
+
+
 def test_device_entry_points(gpu):
     import torch
     img, amri, act = pct.synth_tissue(40, 66, 30, 3, 2, False, 21)
