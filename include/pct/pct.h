@@ -47,14 +47,20 @@ typedef enum pct_status {
 } pct_status;
 
 /* EXEC_CHOICE of the GPU program (src/GPU/PatchBasedPseudoCT.cu:71-78).
- * GM and GM2 run the direct kernel (one lane per voxel, 27-term SSD per
- * candidate); SM runs the LDS-tiled kernel with separable box-sum reuse.
- * All three compute the same function.  GPU_TEST_TIME is a CLI mode. */
+ * The reference's three kernels compute the same function and differ in
+ * how they use the memory hierarchy (and in whether the patch difference is
+ * rounded to float before widening, <= 1 float ulp per term).  Here GM, GM2
+ * and SM all run the LDS-tiled kernel with separable box-sum reuse, in FP32,
+ * for NH up to 15.  DIRECT has no counterpart in the reference (value 3 is
+ * unhandled there, cu:2026-2205): the one-lane-per-voxel kernel, 27-term
+ * SSD per candidate, an independent implementation used for cross-checks;
+ * NH above 15 runs it for every choice.  GPU_TEST_TIME is a CLI mode. */
 typedef enum pct_exec_choice {
     PCT_CHOICE_GM = 0,
     PCT_CHOICE_GM2 = 1,
     PCT_CHOICE_SM = 2,
-    PCT_CHOICE_GPU_TEST_TIME = 4
+    PCT_CHOICE_GPU_TEST_TIME = 4,
+    PCT_CHOICE_DIRECT = 5
 } pct_exec_choice;
 
 typedef struct pct_problem {
@@ -92,10 +98,10 @@ int pct_normalize_labels_dev(float* d_label, size_t n, pct_stream_t stream);
  * (four 16 MiB buffers, filled and emptied by the OpenMP threads).  The
  * device buffers and the pinned buffers stay allocated for the next
  * host-buffer call (grown on demand; pct_release frees them); host-buffer
- * calls on one device run one at a time.  `out` receives every voxel: interior = fused value (NaN where
- * |P| or |S| <= DBL_EPSILON, cpp:1437-1443), border = 0 (cu:1693).
- * `choice` is a pct_exec_choice (GM, GM2 or SM).  The output equals
- * pct_fuse_dev's bit for bit. */
+ * calls on one device run one at a time.  `out` receives every voxel:
+ * interior = fused value (NaN where |P| or |S| <= DBL_EPSILON,
+ * cpp:1437-1443), border = 0 (cu:1693).  `choice` is a pct_exec_choice
+ * (GM, GM2, SM or DIRECT).  The output equals pct_fuse_dev's bit for bit. */
 int pct_fuse_host(const pct_problem* pb, const float* img, const float* atlas,
                   const float* atlas_label_norm, float* out, int choice);
 

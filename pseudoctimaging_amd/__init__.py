@@ -3,7 +3,7 @@
 The fusion runs in hand-written HIP kernels inside libpct.so (C ABI in
 include/pct/pct.h); this package is the Python host interface over it.
 """
-from .api import (CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, autotune,  # noqa: F401
+from .api import (CHOICE_DIRECT, CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, autotune,  # noqa: F401
                   autotune_batch, batch_choice, fuse_batch, set_batch_choice,
                   set_batch_config,
                   device_count, force_tile_config, fuse_device, fused_to_hu_raw,
@@ -15,7 +15,7 @@ from .api import (CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, autotune,  # noqa:
                   tile_config, to_hu, weight_divisor)
 
 __all__ = [
-    "CHOICE_GM", "CHOICE_GM2", "CHOICE_SM", "PctError", "autotune", "autotune_batch",
+    "CHOICE_DIRECT", "CHOICE_GM", "CHOICE_GM2", "CHOICE_SM", "PctError", "autotune", "autotune_batch",
     "batch_choice", "fuse_batch", "set_batch_choice", "set_batch_config", "device_count",
     "force_tile_config", "fuse_device", "fused_to_hu_raw", "mat_read", "mat_write",
     "matrixio_read", "matrixio_write", "matrixio_write_sidecar",

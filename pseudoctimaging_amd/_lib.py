@@ -28,6 +28,7 @@ PCT_ERR_TOO_LARGE = -9
 CHOICE_GM = 0
 CHOICE_GM2 = 1
 CHOICE_SM = 2
+CHOICE_DIRECT = 5   # the one-lane-per-voxel kernel (no reference counterpart)
 CHOICE_GPU_TEST_TIME = 4
 
 

@@ -28,7 +28,7 @@ from ctypes import byref, c_int
 import numpy as np
 
 from . import _lib
-from ._lib import (CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, Problem,  # noqa: F401
+from ._lib import (CHOICE_DIRECT, CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, Problem,  # noqa: F401
                    TileConfig, check, fptr, lib)
 
 

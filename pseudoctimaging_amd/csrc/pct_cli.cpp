@@ -142,7 +142,8 @@ int main(int argc, char** argv) {
     std::vector<float> out(N, 0.0f);
     int status = EXIT_SUCCESS;
 
-    if (choice == PCT_CHOICE_GM || choice == PCT_CHOICE_GM2 || choice == PCT_CHOICE_SM) {
+    if (choice == PCT_CHOICE_GM || choice == PCT_CHOICE_GM2 || choice == PCT_CHOICE_SM ||
+        choice == PCT_CHOICE_DIRECT) {
         // timed region as the reference's cu:2152-2188: fusion incl. copies,
         // plus the post-processing (on the device, bit-identical to the host's)
         const double t0 = now_ms();

@@ -432,8 +432,15 @@ size_t tiled_ws_bytes(const pct_problem* pb, int n_groups) {
     return ((2 * (size_t)n_groups * N * sizeof(float)) + 255) & ~(size_t)255;
 }
 
+static bool valid_choice(int choice) {
+    return choice == PCT_CHOICE_GM || choice == PCT_CHOICE_GM2 || choice == PCT_CHOICE_SM ||
+           choice == PCT_CHOICE_DIRECT;
+}
+
+// GM, GM2 and SM run the tiled kernel where it is instantiated; DIRECT and
+// other window sizes run the direct kernel.
 static bool is_tiled_choice(const pct_problem* pb, int choice) {
-    return choice == PCT_CHOICE_SM && (pb->nh - 1) / 2 <= kMaxTiledHalfWindow &&
+    return choice != PCT_CHOICE_DIRECT && (pb->nh - 1) / 2 <= kMaxTiledHalfWindow &&
            (pb->nh - 1) / 2 >= 1;
 }
 
@@ -628,16 +635,11 @@ static int fuse_dev_impl(const pct_problem* pb, const float* img, const float* a
     int rc = validate_problem(pb);
     if (rc) return rc;
     if (!img || !atlas || !label || !out) return PCT_ERR_INVALID;
-    if (choice != PCT_CHOICE_GM && choice != PCT_CHOICE_GM2 && choice != PCT_CHOICE_SM)
-        return PCT_ERR_INVALID;
+    if (!valid_choice(choice)) return PCT_ERR_INVALID;
     int dev = 0;
     rc = check_device(&dev);
     if (rc) return rc;
-    if (!is_tiled_choice(pb, choice)) {
-        if (choice == PCT_CHOICE_SM && (pb->nh - 1) / 2 > kMaxTiledHalfWindow)
-            return launch_direct(pb, img, atlas, label, out, stream);
-        return launch_direct(pb, img, atlas, label, out, stream);
-    }
+    if (!is_tiled_choice(pb, choice)) return launch_direct(pb, img, atlas, label, out, stream);
     int sid = 0, grp = 1;
     TileGeometry g;
     if (!choose(pb, &sid, &grp, nullptr, &g)) return PCT_ERR_UNSUPPORTED;
@@ -830,8 +832,7 @@ static int fuse_host_impl(const pct_problem* pb, const float* img, const float* 
     int rc = validate_problem(pb);
     if (rc) return rc;
     if (!img || !atlas || !label || !out) return PCT_ERR_INVALID;
-    if (choice != PCT_CHOICE_GM && choice != PCT_CHOICE_GM2 && choice != PCT_CHOICE_SM)
-        return PCT_ERR_INVALID;
+    if (!valid_choice(choice)) return PCT_ERR_INVALID;
     const size_t N = (size_t)pb->nx * pb->ny * pb->nz;
     if (post && N >= (size_t)INT_MAX) return PCT_ERR_TOO_LARGE;   // device post chain's limit
     int dev = 0;

@@ -139,7 +139,7 @@ def test_cfg1_full_volume(gpu):
     lab = act.copy()
     pct.normalize_labels(lab)
     ref = oracle.fuse(img, amri, lab, g["nh"], g["sigma"], g["beta"])
-    for choice in (pct.CHOICE_SM, pct.CHOICE_GM):
+    for choice in (pct.CHOICE_SM, pct.CHOICE_DIRECT):
         out = pct.multi_patch_based_segmentation_gpu(img, amri, lab, g["nh"], g["sigma"],
                                                      g["beta"], choice)
         compare(out, ref)
@@ -165,7 +165,8 @@ def _final_hu_agrees(fused_gpu, fused_ref, hu_ref, nh):
 
 
 @pytest.mark.parametrize("name", ["small_tissue.npz", "small_uniform.npz"])
-@pytest.mark.parametrize("choice", [pct.CHOICE_SM, pct.CHOICE_GM, pct.CHOICE_GM2])
+@pytest.mark.parametrize("choice", [pct.CHOICE_SM, pct.CHOICE_GM, pct.CHOICE_GM2,
+                                    pct.CHOICE_DIRECT])
 def test_golden_small_through_pipeline(gpu, name, choice):
     g = load(name)
     nh = int(g["params"][4])
@@ -197,7 +198,7 @@ def test_edge_cases(gpu, nx, ny, nz, n, nh, sigma, beta):
     lab = act.copy()
     pct.normalize_labels(lab)
     ref = oracle.fuse(img, amri, lab, nh, sigma, beta)
-    for choice in (pct.CHOICE_SM, pct.CHOICE_GM):
+    for choice in (pct.CHOICE_SM, pct.CHOICE_DIRECT):
         out = pct.multi_patch_based_segmentation_gpu(img, amri, lab, nh, sigma, beta, choice)
         compare(out, ref)
     if 2 * ((nh - 1) // 2 + 1) >= min(nx, ny, nz):
@@ -210,7 +211,7 @@ def test_uniform_random_mode_with_nan(gpu):
     img, atlas, label = pct.synth_uniform(20, 22, 18, 2, 99)
     for sigma in (1.0, 0.5, 0.33, 0.25):
         ref = oracle.fuse(img, atlas, label, 3, sigma, 1.0)
-        for choice in (pct.CHOICE_SM, pct.CHOICE_GM):
+        for choice in (pct.CHOICE_SM, pct.CHOICE_DIRECT):
             out = pct.multi_patch_based_segmentation_gpu(img, atlas, label, 3, sigma, 1.0, choice)
             st = compare(out, ref)
             print(sigma, choice, st)
@@ -223,7 +224,8 @@ def test_uniform_random_mode_with_nan(gpu):
     (36, 40, 28, 10, 7, (11, 3), pct.CHOICE_SM),      # groups of 3 atlases, the last short
     (30, 34, 26, 5, 11, (13, 5), pct.CHOICE_SM),      # one group: one chunk
     (30, 34, 26, 2, 17, None, pct.CHOICE_SM),         # no tiled kernel: direct path
-    (30, 34, 26, 2, 9, None, pct.CHOICE_GM2),
+    (30, 34, 26, 2, 9, None, pct.CHOICE_DIRECT),      # the direct kernel by choice
+    (30, 34, 26, 2, 9, None, pct.CHOICE_GM2),         # GM2 runs the tiled kernel
 ])
 def test_host_call_overlapped_copies_equal_device_call(gpu, case):
     """pct_fuse_host copies the atlases in chunks that overlap the fusion; its
@@ -251,7 +253,7 @@ def test_host_call_overlapped_copies_equal_device_call(gpu, case):
         pct.force_tile_config(-1, 0)
 
 
-@pytest.mark.parametrize("choice", [pct.CHOICE_SM, pct.CHOICE_GM2])
+@pytest.mark.parametrize("choice", [pct.CHOICE_SM, pct.CHOICE_DIRECT])
 def test_host_call_staging_of_large_buffers(gpu, choice):
     """Host buffers larger than the pinned staging ring (four 16 MiB pieces):
     75 MB volumes go over in five pieces each, in and out, and the output

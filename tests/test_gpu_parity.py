@@ -29,7 +29,7 @@ SMALL = [
 
 
 @pytest.mark.parametrize("nx,ny,nz,n,nh,sigma,beta", SMALL)
-@pytest.mark.parametrize("choice", [pct.CHOICE_SM, pct.CHOICE_GM2])
+@pytest.mark.parametrize("choice", [pct.CHOICE_SM, pct.CHOICE_DIRECT])
 def test_plain_vs_oracle(gpu, nx, ny, nz, n, nh, sigma, beta, choice):
     print(f"case {nx}x{ny}x{nz} n={n} nh={nh} choice={choice}", flush=True)
     img, amri, lab = _case(nx, ny, nz, n, seed=nx * 7 + nh)
