@@ -415,7 +415,7 @@ def main() -> int:
 
     e2e = None
     if not args.no_end_to_end:
-        log("end-to-end leg: host buffers, per-call device memory, host post chain")
+        log("end-to-end leg: host buffers, pinned staging, host post chain")
         e2e = end_to_end_leg(img, amri, lab, nh, sigma, beta, 3)
         log(f"end-to-end: {e2e['total_ms']:.1f} ms (fusion call {e2e['fuse_host_ms']:.1f} ms, "
             f"post {e2e['post_host_ms']:.1f} ms); pct_pseudo_ct_host "
