@@ -583,10 +583,14 @@ __global__ __launch_bounds__(kBoxThreads) void box_xyz(const float* __restrict__
     constexpr int NLD = (R * C + kBoxThreads - 1) / kBoxThreads;   // tile loads per thread
     constexpr int NYT = R * (kBoxTY / 4);        // y-sum tasks
     constexpr int OX = kBoxTX / (kBoxThreads / 64);   // x outputs per thread
+    // row pitch of the y sums: a multiple of 4 for the 128-bit stores, and
+    // not of 64, so the column reads of the x sums (two rows per
+    // instruction) hit different banks
+    constexpr int YP = kBoxTY + 4;
     static_assert(CP >= C && CP % 4 == 0, "row pitch");
     static_assert(NYT <= kBoxThreads && OX == 2, "one y-sum task and 2 outputs per thread");
     __shared__ __align__(16) float T[R * CP];
-    __shared__ __align__(16) float Y[R * kBoxTY];
+    __shared__ __align__(16) float Y[R * YP];
 
     const int tiles_x = (nx + kBoxTX - 1) / kBoxTX, tiles_y = (ny + kBoxTY - 1) / kBoxTY;
     const int runs = (nz + kBoxRun - 1) / kBoxRun;
@@ -664,12 +668,12 @@ __global__ __launch_bounds__(kBoxThreads) void box_xyz(const float* __restrict__
                         for (int k = 0; k < K; ++k) sacc += w4[m + k];
                         op[m] = sacc;
                     }
-                    *reinterpret_cast<float4*>(&Y[rr * kBoxTY + c0]) = o;
+                    *reinterpret_cast<float4*>(&Y[rr * YP + c0]) = o;
                 }
                 __syncthreads();
                 float col[OX + 2 * HW];
 #pragma unroll
-                for (int k = 0; k < OX + 2 * HW; ++k) col[k] = Y[(ox + k) * kBoxTY + oy];
+                for (int k = 0; k < OX + 2 * HW; ++k) col[k] = Y[(ox + k) * YP + oy];
 #pragma unroll
                 for (int m = 0; m < OX; ++m) {
                     float sacc = 0.f;
