@@ -442,7 +442,55 @@ struct ModelCache {
 };
 static ModelCache g_model;
 
+// Second stream of the pruned call, per device, created on first use at the
+// default priority: the CT window sums and the scaled atlas copy depend on
+// neither the embeddings nor the selection and run beside them.  The fork
+// and join events are recorded and waited on under `mu`, so calls from
+// several host threads cannot re-record one between its record and its wait.
+struct SideStreams {
+    std::mutex mu;
+    hipStream_t st[64] = {};
+    hipEvent_t fork[64] = {}, join[64] = {};
+};
+static SideStreams g_side;
+
+static int side_stream(int dev) {   // caller holds g_side.mu
+    if (dev < 0 || dev >= 64) return PCT_ERR_NO_DEVICE;
+    if (g_side.st[dev]) return PCT_OK;
+    hipStream_t st = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&join, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        if (join) (void)hipEventDestroy(join);
+        if (fork) (void)hipEventDestroy(fork);
+        if (st) (void)hipStreamDestroy(st);
+        return PCT_ERR_HIP;
+    }
+    g_side.st[dev] = st;   // all three, or none
+    g_side.fork[dev] = fork;
+    g_side.join[dev] = join;
+    return PCT_OK;
+}
+
 void release_learned_models() {
+    {
+        std::lock_guard<std::mutex> lk(g_side.mu);
+        for (int dev = 0; dev < 64; ++dev) {
+            if (!g_side.st[dev]) continue;
+            int cur = 0;
+            (void)hipGetDevice(&cur);
+            (void)hipSetDevice(dev);
+            (void)hipStreamSynchronize(g_side.st[dev]);
+            (void)hipStreamDestroy(g_side.st[dev]);
+            if (g_side.fork[dev]) (void)hipEventDestroy(g_side.fork[dev]);
+            if (g_side.join[dev]) (void)hipEventDestroy(g_side.join[dev]);
+            (void)hipSetDevice(cur);
+            g_side.st[dev] = nullptr;
+            g_side.fork[dev] = g_side.join[dev] = nullptr;
+        }
+    }
     std::lock_guard<std::mutex> lk(g_model.mu);
     for (int dev = 0; dev < 64; ++dev) {
         if (!g_model.d_model[dev]) continue;
@@ -1504,6 +1552,41 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
             if (need[(size_t)z]) zlist.push_back(z);
     }
 
+    // the pruned kernel's arguments (problem, tiles, partials) and the two
+    // inputs it reads besides the selection: the window sums of every atlas
+    // CT and the atlas MRI pre-scaled for the direct LDS loads.  Those two
+    // run on the second stream, beside steps 1 and 3.
+    float* part = reinterpret_cast<float*>(w + o_part);
+    PrunedArgs pa;
+    std::memset(&pa, 0, sizeof(pa));
+    TileGeometry g1 = g;
+    g1.group = 1;
+    g1.n_groups = n;
+    g1.n_items = items;
+    pa.t = make_tiled_args(pb, g1, d_img, d_atlas, d_label, part);
+    float* boxL = reinterpret_cast<float*>(w + o_boxL);
+    float* scaled = reinterpret_cast<float*>(w + o_scaled);
+    std::unique_lock<std::mutex> side_lk(g_side.mu);   // until the join is enqueued on s
+    rc = side_stream(dev);
+    if (rc) return rc;
+    hipStream_t ss = g_side.st[dev];
+    if (hipEventRecord(g_side.fork[dev], s) != hipSuccess ||
+        hipStreamWaitEvent(ss, g_side.fork[dev], 0) != hipSuccess)
+        return PCT_ERR_HIP;
+    {
+        // 2. window sums of every atlas CT
+        const long long blocks = (long long)n * ((pb->nz + kBoxRun - 1) / kBoxRun) *
+                                 ((pb->nx + kBoxTX - 1) / kBoxTX) *
+                                 ((pb->ny + kBoxTY - 1) / kBoxTY);
+        hipLaunchKernelGGL(kBoxXyz[hw - 1], dim3((unsigned)blocks), dim3(kBoxThreads), 0, ss,
+                           d_label, boxL, pb->nx, pb->ny, pb->nz, n);
+        const unsigned bs = (unsigned)std::min<long long>((N * n + 255) / 256, 65536);
+        hipLaunchKernelGGL(scale_copy, dim3(bs), dim3(256), 0, ss, d_atlas, scaled, N * n,
+                           pa.t.scale);
+        if (hipGetLastError() != hipSuccess || hipEventRecord(g_side.join[dev], ss) != hipSuccess)
+            return PCT_ERR_HIP;
+    }
+
     // 1. embeddings of the target and of every atlas MRI
     unsigned short* embT = reinterpret_cast<unsigned short*>(w + o_embT);
     unsigned short* embA = reinterpret_cast<unsigned short*>(w + o_embA);
@@ -1526,17 +1609,6 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
                                       embA, s, d_zlist, (int)zlist.size());
     }
     if (rc) return rc;
-
-    // 2. window sums of every atlas CT
-    float* boxL = reinterpret_cast<float*>(w + o_boxL);
-    {
-        const long long blocks = (long long)n * ((pb->nz + kBoxRun - 1) / kBoxRun) *
-                                 ((pb->nx + kBoxTX - 1) / kBoxTX) *
-                                 ((pb->ny + kBoxTY - 1) / kBoxTY);
-        hipLaunchKernelGGL(kBoxXyz[hw - 1], dim3((unsigned)blocks), dim3(kBoxThreads), 0, s, d_label, boxL,
-                           pb->nx, pb->ny, pb->nz, n);
-        if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
-    }
 
     // 3. selection
     SelectArgs sa;
@@ -1582,22 +1654,10 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
         if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
     }
 
-    // 4. exact weighting of the kept candidates and the lattice samples
-    float* part = reinterpret_cast<float*>(w + o_part);
-    PrunedArgs pa;
-    std::memset(&pa, 0, sizeof(pa));
-    TileGeometry g1 = g;
-    g1.group = 1;
-    g1.n_groups = n;
-    g1.n_items = items;
-    pa.t = make_tiled_args(pb, g1, d_img, d_atlas, d_label, part);
-    float* scaled = reinterpret_cast<float*>(w + o_scaled);
-    {
-        const unsigned bs = (unsigned)std::min<long long>((N * n + 255) / 256, 65536);
-        hipLaunchKernelGGL(scale_copy, dim3(bs), dim3(256), 0, s, d_atlas, scaled, N * n,
-                           pa.t.scale);
-        if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
-    }
+    // 4. exact weighting of the kept candidates and the lattice samples,
+    // after the second stream's work
+    if (hipStreamWaitEvent(s, g_side.join[dev], 0) != hipSuccess) return PCT_ERR_HIP;
+    side_lk.unlock();
     pa.atlas_scaled = scaled;
     pa.sel = sa.sel;
     pa.flag = sa.flag;
