@@ -273,7 +273,11 @@ typedef struct pct_pruned_opts {
  * with a low selection margin run the full exact search.  Output as
  * pct_fuse_dev.  If fallback_frac is not NULL the call synchronises and
  * stores the fraction of (tile, atlas) items that fell back.  Planes of
- * nx*ny >= 2^31 voxels return PCT_ERR_TOO_LARGE. */
+ * nx*ny >= 2^31 voxels return PCT_ERR_TOO_LARGE.  Part of the work runs on
+ * a second stream of the library's, forked from and joined back to
+ * `stream` with events: all of it comes after the work already queued on
+ * `stream` and before the work queued after the call, and the call can be
+ * captured into a graph. */
 int pct_fuse_pruned_dev(const pct_problem* pb, const float* params, const pct_pruned_opts* opts,
                         const float* d_img, const float* d_atlas,
                         const float* d_atlas_label_norm, float* d_out, double* fallback_frac,
