@@ -276,8 +276,8 @@ typedef struct pct_pruned_opts {
  * nx*ny >= 2^31 voxels return PCT_ERR_TOO_LARGE.  Part of the work runs on
  * a second stream of the library's, forked from and joined back to
  * `stream` with events: all of it comes after the work already queued on
- * `stream` and before the work queued after the call, and the call can be
- * captured into a graph. */
+ * `stream` and before the work queued after the call.  The call copies
+ * small tables from host memory, so it is not meant for graph capture. */
 int pct_fuse_pruned_dev(const pct_problem* pb, const float* params, const pct_pruned_opts* opts,
                         const float* d_img, const float* d_atlas,
                         const float* d_atlas_label_norm, float* d_out, double* fallback_frac,
