@@ -785,8 +785,10 @@ struct SelectArgs {
     unsigned score_bits;          // dz indices (dz + hw) whose offsets are scored
 };
 
-constexpr int kSelThreads = 512;
-constexpr int kSelPerThread = 8;    // 4096 offsets (NH = 15: 3375)
+constexpr int kSelThreads = 256;
+constexpr int kSelPerThread = 16;   // 4096 offsets (NH = 15: 3375)
+static_assert(kSelThreads >= 16 * 15, "16 threads per dz slab in the slab minima");
+static_assert(kSelThreads >= 15 * 15, "one thread per (dx, dy) in the gathers");
 constexpr int kNoOffset = 1 << 30;
 
 // One workgroup per item = (atlas, tile), item = atlas * n_tiles + tile.
@@ -980,7 +982,7 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
     const unsigned short* __restrict__ eA = a.embA + (long long)at * a.N * 16;
     // B operands of one plane: wave wv takes tiles wv, wv + 8, ...; lane
     // holds dims 4 lg .. 4 lg + 3 of voxel 16 tile + ln
-    constexpr int kTilesPerWave = 3;   // 18 tiles at NH = 15
+    constexpr int kTilesPerWave = 5;   // 18 tiles at NH = 15 over 4 waves
     // in-plane element offset of the lane's window voxel per tile (the same
     // on every plane), -1 past the window
     long long xyo[kTilesPerWave];
