@@ -965,7 +965,7 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
     // consecutive planes lives in an LDS ring; the offsets of one dz read the
     // plane of the z = 0 probes and the next one.
     float* key = sh;                                   // [ncand] scores by offset
-    float* ring = sh + ((ncand + 63) & ~63);           // [2][16][pvs]
+    float* ring = sh + ((ncand + 63) & ~63);           // [16][pvs] G of one window plane
     v4s aop;   // A operand: lane holds -2 t[p = ln][k = 4 lg .. 4 lg + 3]
     {
         const int yy = min(py0 + (ln & 3), a.ny - 1), xx = min(px0 + ((ln >> 2) & 1), a.nx - 1),
@@ -1028,9 +1028,13 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
         }
     };
     // Scores are computed for the dz planes in a.score_bits (every plane, or
-    // only those kept offsets may come from); a window plane is computed
-    // when one of them reads it, and the loads of the next such plane are in
-    // flight during each round of gathers.
+    // only those kept offsets may come from).  Window plane wz holds the
+    // voxels that the probes with z = 0 read for dz index wz and the probes
+    // with z = 1 read for dz index wz - 1, so a plane is computed when one of
+    // those is scored, and one pass over its G finishes the score of dz
+    // index wz - 1 and starts that of wz.  A gathering thread keeps its
+    // offset's four partial sums in registers from one plane to the next.
+    // The loads of the next such plane are in flight during the gathers.
     auto scored = [&](int dzi) { return dzi >= 0 && dzi < nw && ((a.score_bits >> dzi) & 1u); };
     auto plane_needed = [&](int wz) { return scored(wz) || scored(wz - 1); };
     uint2 bq[kTilesPerWave];
@@ -1041,37 +1045,38 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
             loaded = wz;
         }
     };
-    if (plane_needed(0)) {
-        ensure(0);
+    const bool gatherer = (int)threadIdx.x < nw2;
+    const int dxi = gatherer ? (int)threadIdx.x / nw : 0;
+    const int dyi = gatherer ? (int)threadIdx.x - dxi * nw : 0;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int wz = 0; wz < WZ; ++wz) {
+        if (!plane_needed(wz)) continue;   // uniform over the workgroup
+        ensure(wz);
         plane_g(0, bq);
-    }
-    if (plane_needed(1)) ensure(1);
-    for (int dzi = 0; dzi < nw; ++dzi) {
-        const bool np = plane_needed(dzi + 1), sc_now = scored(dzi);
-        if (!np && !sc_now) continue;   // uniform over the workgroup
-        if (np) {
-            ensure(dzi + 1);
-            plane_g((dzi + 1) & 1, bq);
-        }
-        for (int w2 = dzi + 2; w2 < WZ; ++w2)
+        for (int w2 = wz + 1; w2 < WZ; ++w2)
             if (plane_needed(w2)) {
                 ensure(w2);   // in flight during the gathers
                 break;
             }
         __syncthreads();
-        if (sc_now && (int)threadIdx.x < nw2) {
-            const int dxi = threadIdx.x / nw, dyi = threadIdx.x - dxi * nw;
-            const float* g0 = ring + (dzi & 1) * 16 * pvs;         // probes with z = 0
-            const float* g1 = ring + ((dzi + 1) & 1) * 16 * pvs;   // probes with z = 1
-            float acc[4] = {0.f, 0.f, 0.f, 0.f};
+        if (gatherer) {
+            if (scored(wz - 1)) {
+                // probes with z = 1 of dz index wz - 1
 #pragma unroll
-            for (int p = 0; p < 16; ++p) {
-                const float* g = (p >> 3) ? g1 : g0;
-                acc[p & 3] += g[p * pvs + (((p >> 2) & 1) + dxi) * WY + (p & 3) + dyi];
+                for (int p = 8; p < 16; ++p)
+                    acc[p & 3] += ring[p * pvs + (((p >> 2) & 1) + dxi) * WY + (p & 3) + dyi];
+                key[(wz - 1) * nw2 + threadIdx.x] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
             }
-            key[dzi * nw2 + threadIdx.x] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            if (scored(wz)) {
+                // probes with z = 0 of dz index wz
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[j] = 0.f;
+#pragma unroll
+                for (int p = 0; p < 8; ++p)
+                    acc[p & 3] += ring[p * pvs + (((p >> 2) & 1) + dxi) * WY + (p & 3) + dyi];
+            }
         }
-        __syncthreads();   // the ring slot is rewritten next
+        __syncthreads();   // the ring is rewritten next
     }
     float sc[kSelPerThread];
 #pragma unroll
@@ -1643,11 +1648,11 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
                           : std::min(nw, std::max(opts->max_slabs, sa.n_lattice_dz));
     int p2 = 1;
     while (p2 < slabs * nw * nw) p2 <<= 1;
-    // scores by offset, then either the G ring of two window planes (while
-    // scoring) or the sort arrays (after it)
+    // scores by offset, then either G of one window plane (while scoring)
+    // or the sort arrays (after it)
     const int ring_stride = (((4 + 2 * hw) * (2 + 2 * hw) + 15) / 16) * 16;
     const size_t sel_lds = (size_t)((ncand + 63) & ~63) * 4 +
-                           std::max((size_t)2 * 16 * ring_stride * 4, (size_t)p2 * 8);
+                           std::max((size_t)16 * ring_stride * 4, (size_t)p2 * 8);
     if (sel_lds > (size_t)kLdsPerCu) return PCT_ERR_UNSUPPORTED;
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(select_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sel_lds);
