@@ -1486,6 +1486,7 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     const int n = pb->n_atlas;
     const long long N = (long long)pb->nx * pb->ny * pb->nz;
     const long long items = g.n_tiles * n;
+    if (items >= INT_MAX) return PCT_ERR_TOO_LARGE;   // one workgroup per item, int item index
     const int ncand = (2 * hw + 1) * (2 * hw + 1) * (2 * hw + 1);
     const int topk = std::min(opts->topk, ncand);
 
