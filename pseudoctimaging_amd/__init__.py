@@ -11,7 +11,8 @@ from .api import (CHOICE_DIRECT, CHOICE_GM, CHOICE_GM2, CHOICE_SM, PctError, aut
                   multi_patch_based_segmentation_gpu,
                   normalize_labels, normalize_labels_device, postprocess, postprocess_device,
                   problem, pseudo_ct, pseudo_ct_hu, regularization_step_device,
-                  read_config, regularization_step, round_matrix, synth_tissue, synth_uniform,
+                  read_config, regularization_step, release, round_matrix, synth_tissue,
+                  synth_uniform,
                   tile_config, to_hu, weight_divisor)
 
 __all__ = [
@@ -21,6 +22,6 @@ __all__ = [
     "matrixio_read", "matrixio_write", "matrixio_write_sidecar",
     "multi_patch_based_segmentation_gpu", "normalize_labels", "normalize_labels_device",
     "postprocess", "postprocess_device", "problem", "pseudo_ct", "pseudo_ct_hu", "read_config",
-    "regularization_step", "regularization_step_device", "round_matrix",
+    "regularization_step", "regularization_step_device", "release", "round_matrix",
     "synth_tissue", "synth_uniform", "tile_config", "to_hu", "weight_divisor",
 ]

@@ -60,6 +60,13 @@ def normalize_labels(label: np.ndarray) -> np.ndarray:
     return label
 
 
+def release() -> None:
+    """pct_release: frees the library's per-device workspaces, cached host-call
+    buffers (device and pinned), streams, batch graphs and model copies.  The
+    next call recreates what it needs."""
+    lib().pct_release()
+
+
 def device_count() -> int:
     return lib().pct_device_count()
 
