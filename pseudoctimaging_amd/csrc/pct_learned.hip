@@ -1208,11 +1208,66 @@ struct PrunedArgs {
     const int* sel;                 // [items][topk]
     const unsigned char* flag;      // [items]
     const float* boxL;              // [n][N] window sums of the CT
-    int topk;
-    int n_samples;
-    int samples[kSamples];          // lattice offset indices, ascending
+    const int* merged;              // [items][kMergedStride] offset lists (merge_offsets_kernel)
     int ncand;                      // NH^3
 };
+
+// Per (tile, atlas) item, the offsets the pruned kernel evaluates, in its
+// order: per dz plane (ascending), the kept offsets, then the lattice samples
+// that are not kept.  One thread per item, the block's kept lists staged in
+// LDS.  Entry 0 holds the count and the number of samples; entry 1 + k holds
+// offset k as (sample << 31) | ((dz + 8) << 16) | LDS delta (dy + dx rows of
+// the slab, row length cw).
+constexpr int kMergeThreads = 64;
+constexpr int kMergedStride = 96;   // >= 1 + kMaxOffsets
+static_assert(kMergedStride >= 1 + kMaxOffsets, "merged list stride");
+
+struct MergeArgs {
+    const int* sel;                 // [items][topk] kept offsets, ascending, kNoOffset after
+    int* merged;                    // [items][kMergedStride]
+    long long items;
+    int topk, hw, cw;
+    int n_samples;
+    int samples[kSamples];          // lattice offset indices, ascending
+};
+
+__global__ __launch_bounds__(kMergeThreads) void merge_offsets_kernel(MergeArgs a) {
+    __shared__ int kept_s[kMergeThreads][kMaxTopk + 1];
+    const long long item0 = (long long)blockIdx.x * kMergeThreads;
+    const int nit = (int)min((long long)kMergeThreads, a.items - item0);
+    for (int e = threadIdx.x; e < nit * a.topk; e += kMergeThreads) {
+        const int it = e / a.topk;
+        kept_s[it][e - it * a.topk] = a.sel[item0 * a.topk + e];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x >= nit) return;
+    const int* kept = kept_s[threadIdx.x];
+    int* out = a.merged + (item0 + threadIdx.x) * kMergedStride;
+    const int nw1 = 2 * a.hw + 1, nw2 = nw1 * nw1;
+    int i = 0, j = 0, m = 0, ns = 0;
+    auto push = [&](int o, unsigned smpl) {
+        const int dz = o / nw2 - a.hw;
+        const int del = ((o / nw1) % nw1) * a.cw + o % nw1;
+        out[1 + m] = (int)((smpl << 31) | ((unsigned)(dz + 8) << 16) | (unsigned)del);
+        ++m;
+    };
+    for (int d = 0; d < nw1; ++d) {
+        const int hi = (d + 1) * nw2;
+        const int i0 = i;
+        while (i < a.topk && kept[i] < hi) push(kept[i++], 0u);
+        // a sample that is also kept counts once, as kept
+        int t = i0;
+        while (j < a.n_samples && a.samples[j] < hi) {
+            while (t < i && kept[t] < a.samples[j]) ++t;
+            if (!(t < i && kept[t] == a.samples[j])) {
+                push(a.samples[j], 1u);
+                ++ns;
+            }
+            ++j;
+        }
+    }
+    out[0] = m | (ns << 16);
+}
 
 // As tiled_step, also summing the CT of the evaluated candidates (Le) and
 // the weights of the lattice samples (Ws).
@@ -1278,13 +1333,11 @@ template <int W, int BX, int BZ, int NW, int HW>
 __global__ __launch_bounds__(NW * 64) void pruned_kernel(PrunedArgs pa) {
     using C = TiledCfg<W, BX, BZ, NW, HW>;
     extern __shared__ float lds[];
-    // merged offset list: per dz plane (ascending), the kept offsets, then
-    // the lattice samples that are not kept; each entry's dz and LDS delta
+    // the offset list: per dz plane (ascending), the kept offsets, then the
+    // lattice samples that are not kept; each entry's dz and LDS delta
     __shared__ int odz[kMaxOffsets];
     __shared__ int odel[kMaxOffsets];
     __shared__ int is_sample[kMaxOffsets];
-    __shared__ int kept[kMaxTopk], smp[kSamples];
-    __shared__ int n_offs, n_samp_eff;
     const TiledArgs& args = pa.t;
     const int item = xcd_contiguous(blockIdx.x, args.n_items);
     if (pa.flag[item]) return;            // full search handles it (uniform)
@@ -1295,38 +1348,14 @@ __global__ __launch_bounds__(NW * 64) void pruned_kernel(PrunedArgs pa) {
     const int tx = t % args.tiles_x;
     const int tz = t / args.tiles_x;
 
-    // both lists into LDS with one load per thread, then one thread merges
-    if (threadIdx.x < pa.topk) kept[threadIdx.x] = pa.sel[(long long)item * pa.topk + threadIdx.x];
-    if (threadIdx.x < pa.n_samples) smp[threadIdx.x] = pa.samples[threadIdx.x];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        // both lists are ascending (dz-major).  Per dz plane: its kept
-        // offsets, then its samples; a sample that is also kept counts once,
-        // as kept.  Every step of an inner loop below then runs the same code.
-        constexpr int NW1 = 2 * HW + 1, NW2 = NW1 * NW1;
-        int i = 0, j = 0, m = 0, ns = 0;
-        auto push = [&](int o, int smpl) {
-            odz[m] = o / NW2 - HW;
-            odel[m] = ((o / NW1) % NW1) * C::CW + o % NW1;
-            is_sample[m] = smpl;
-            ++m;
-        };
-        for (int d = 0; d < NW1; ++d) {
-            const int hi = (d + 1) * NW2;
-            const int i0 = i;
-            while (i < pa.topk && kept[i] < hi) push(kept[i++], 0);
-            int t = i0;
-            while (j < pa.n_samples && smp[j] < hi) {
-                while (t < i && kept[t] < smp[j]) ++t;
-                if (!(t < i && kept[t] == smp[j])) {
-                    push(smp[j], 1);
-                    ++ns;
-                }
-                ++j;
-            }
-        }
-        n_offs = m;
-        n_samp_eff = ns;
+    const int* __restrict__ mo = pa.merged + (long long)item * kMergedStride;
+    const int hdr = mo[0];
+    const int n_offs = hdr & 0xffff, n_samp_eff = hdr >> 16;
+    if ((int)threadIdx.x < n_offs) {   // n_offs <= kMaxOffsets < NW * 64
+        const unsigned e = (unsigned)mo[1 + threadIdx.x];
+        odz[threadIdx.x] = (int)((e >> 16) & 15u) - 8;
+        odel[threadIdx.x] = (int)(e & 0xffffu);
+        is_sample[threadIdx.x] = (int)(e >> 31);
     }
     __syncthreads();
 
@@ -1499,6 +1528,7 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     const size_t o_boxL = off; off = align(off + sizeof(float) * (size_t)N * n);
     const size_t o_sel = off; off = align(off + sizeof(int) * (size_t)items * topk);
     const size_t o_flag = off; off = align(off + (size_t)items);
+    const size_t o_merged = off; off = align(off + sizeof(int) * (size_t)items * kMergedStride);
     const size_t o_scaled = off; off = align(off + sizeof(float) * (size_t)N * n);
     const size_t o_zlist = off; off = align(off + sizeof(int) * (size_t)pb->nz);
     void* ws = nullptr;
@@ -1670,10 +1700,26 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     pa.sel = sa.sel;
     pa.flag = sa.flag;
     pa.boxL = boxL;
-    pa.topk = topk;
     pa.ncand = ncand;
-    std::memcpy(pa.samples, samples, sizeof(samples));
-    pa.n_samples = n_samples;
+    // the evaluated offsets of every item, merged once (one thread per item)
+    {
+        MergeArgs ma;
+        ma.sel = sa.sel;
+        ma.merged = reinterpret_cast<int*>(w + o_merged);
+        ma.items = items;
+        ma.topk = topk;
+        ma.hw = hw;
+        ma.cw = sh.seg_width + 2 * hw;   // LDS row length of the tiled slab (TiledCfg::CW)
+        ma.n_samples = n_samples;
+        std::memcpy(ma.samples, samples, sizeof(samples));
+        if (items > 0) {
+            hipLaunchKernelGGL(merge_offsets_kernel,
+                               dim3((unsigned)((items + kMergeThreads - 1) / kMergeThreads)),
+                               dim3(kMergeThreads), 0, s, ma);
+            if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
+        }
+        pa.merged = ma.merged;
+    }
     PrunedFn fn = pe->fn[hw - 1];
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                               hipFuncAttributeMaxDynamicSharedMemorySize, g.lds_bytes);
