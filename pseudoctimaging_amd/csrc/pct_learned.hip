@@ -1214,13 +1214,17 @@ struct PrunedArgs {
 
 // Per (tile, atlas) item, the offsets the pruned kernel evaluates, in its
 // order: per dz plane (ascending), the kept offsets, then the lattice samples
-// that are not kept.  One thread per item, the block's kept lists staged in
-// LDS.  Entry 0 holds the count and the number of samples; entry 1 + k holds
-// offset k as (sample << 31) | ((dz + 8) << 16) | LDS delta (dy + dx rows of
-// the slab, row length cw).
-constexpr int kMergeThreads = 64;
+// that are not kept (a sample that is also kept counts once, as kept).  One
+// wave per item, every entry placed by counting: a kept offset goes after the
+// kept offsets before it and the samples of the earlier planes; a sample goes
+// after the kept offsets of its plane and the planes before, and after the
+// samples before it.  Entry 0 holds the count and the number of samples;
+// entry 1 + k holds offset k as (sample << 31) | ((dz + 8) << 16) | LDS delta
+// (dy + dx rows of the slab, row length cw).
+constexpr int kMergeWaves = 4;
 constexpr int kMergedStride = 96;   // >= 1 + kMaxOffsets
 static_assert(kMergedStride >= 1 + kMaxOffsets, "merged list stride");
+static_assert(kMaxTopk <= 64 && kSamples <= 64, "one lane per kept offset and per sample");
 
 struct MergeArgs {
     const int* sel;                 // [items][topk] kept offsets, ascending, kNoOffset after
@@ -1231,42 +1235,39 @@ struct MergeArgs {
     int samples[kSamples];          // lattice offset indices, ascending
 };
 
-__global__ __launch_bounds__(kMergeThreads) void merge_offsets_kernel(MergeArgs a) {
-    __shared__ int kept_s[kMergeThreads][kMaxTopk + 1];
-    const long long item0 = (long long)blockIdx.x * kMergeThreads;
-    const int nit = (int)min((long long)kMergeThreads, a.items - item0);
-    for (int e = threadIdx.x; e < nit * a.topk; e += kMergeThreads) {
-        const int it = e / a.topk;
-        kept_s[it][e - it * a.topk] = a.sel[item0 * a.topk + e];
-    }
-    __syncthreads();
-    if ((int)threadIdx.x >= nit) return;
-    const int* kept = kept_s[threadIdx.x];
-    int* out = a.merged + (item0 + threadIdx.x) * kMergedStride;
+__global__ __launch_bounds__(kMergeWaves * 64) void merge_offsets_kernel(MergeArgs a) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long long item = (long long)blockIdx.x * kMergeWaves + wv;
+    if (item >= a.items) return;   // per wave; no workgroup barrier below
     const int nw1 = 2 * a.hw + 1, nw2 = nw1 * nw1;
-    int i = 0, j = 0, m = 0, ns = 0;
-    auto push = [&](int o, unsigned smpl) {
+    const int kv = lane < a.topk ? a.sel[item * a.topk + lane] : kNoOffset;
+    int sv = kNoOffset;   // the lane's sample (uniform indices into the arguments)
+    for (int j = 0; j < kSamples; ++j)
+        if (j == lane && j < a.n_samples) sv = a.samples[j];
+    const bool has_k = kv < kNoOffset, has_s = sv < kNoOffset;
+    bool dup = false;
+    for (int q = 0; q < a.topk; ++q) dup |= __shfl(kv, q) == sv;
+    const unsigned long long smask = __ballot(has_s && !dup);   // samples evaluated
+    const unsigned long long kmask = __ballot(has_k);
+    const int klo = has_k ? (kv / nw2) * nw2 : 0;          // first offset of the kept plane
+    const int shi = has_s ? (sv / nw2 + 1) * nw2 : 0;      // end of the sample's plane
+    int before_k = 0, before_s = 0;
+    for (int j = 0; j < a.n_samples; ++j)
+        before_k += (int)((smask >> j) & 1ull) & (__shfl(sv, j) < klo);
+    for (int q = 0; q < a.topk; ++q) before_s += __shfl(kv, q) < shi;
+    auto pack = [&](int o, unsigned smpl) {
         const int dz = o / nw2 - a.hw;
         const int del = ((o / nw1) % nw1) * a.cw + o % nw1;
-        out[1 + m] = (int)((smpl << 31) | ((unsigned)(dz + 8) << 16) | (unsigned)del);
-        ++m;
+        return (int)((smpl << 31) | ((unsigned)(dz + 8) << 16) | (unsigned)del);
     };
-    for (int d = 0; d < nw1; ++d) {
-        const int hi = (d + 1) * nw2;
-        const int i0 = i;
-        while (i < a.topk && kept[i] < hi) push(kept[i++], 0u);
-        // a sample that is also kept counts once, as kept
-        int t = i0;
-        while (j < a.n_samples && a.samples[j] < hi) {
-            while (t < i && kept[t] < a.samples[j]) ++t;
-            if (!(t < i && kept[t] == a.samples[j])) {
-                push(a.samples[j], 1u);
-                ++ns;
-            }
-            ++j;
-        }
+    int* out = a.merged + item * kMergedStride;
+    if (has_k) out[1 + lane + before_k] = pack(kv, 0u);
+    if (has_s && !dup)
+        out[1 + before_s + __popcll(smask & ((1ull << lane) - 1ull))] = pack(sv, 1u);
+    if (lane == 0) {
+        const int ns = __popcll(smask);
+        out[0] = (__popcll(kmask) + ns) | (ns << 16);
     }
-    out[0] = m | (ns << 16);
 }
 
 // As tiled_step, also summing the CT of the evaluated candidates (Le) and
@@ -1701,7 +1702,7 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     pa.flag = sa.flag;
     pa.boxL = boxL;
     pa.ncand = ncand;
-    // the evaluated offsets of every item, merged once (one thread per item)
+    // the evaluated offsets of every item, merged once (one wave per item)
     {
         MergeArgs ma;
         ma.sel = sa.sel;
@@ -1714,8 +1715,8 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
         std::memcpy(ma.samples, samples, sizeof(samples));
         if (items > 0) {
             hipLaunchKernelGGL(merge_offsets_kernel,
-                               dim3((unsigned)((items + kMergeThreads - 1) / kMergeThreads)),
-                               dim3(kMergeThreads), 0, s, ma);
+                               dim3((unsigned)((items + kMergeWaves - 1) / kMergeWaves)),
+                               dim3(kMergeWaves * 64), 0, s, ma);
             if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
         }
         pa.merged = ma.merged;
