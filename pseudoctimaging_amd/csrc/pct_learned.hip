@@ -783,6 +783,8 @@ struct SelectArgs {
     int n_lattice_dz;
     int lattice_dz[3];            // dz of the lattice samples (always allowed)
     unsigned score_bits;          // dz indices (dz + hw) whose offsets are scored
+    unsigned fixed_allowed;       // the allowed dz indices when they do not depend on the
+                                  // scores, else 0
 };
 
 constexpr int kSelThreads = 256;
@@ -1084,49 +1086,62 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
         const int c = threadIdx.x + q * kSelThreads;
         sc[q] = c < ncand ? key[c] : __builtin_huge_valf();
     }
-    // best score of every dz slab (16 threads per slab, then a shuffle
-    // reduction), then the allowed slabs: the lattice's and the best others
-    // up to max_slabs (ties to the smaller dz)
-    {
-        const int d = threadIdx.x >> 4, part = threadIdx.x & 15;
-        float m = __builtin_huge_valf();
-        if (d < nw && scored(d))
-            for (int c = d * nw2 + part; c < (d + 1) * nw2; c += 16) m = fminf(m, key[c]);
+    // the allowed slabs: the lattice's and the best others up to max_slabs
+    // (ties to the smaller dz).  Where the host has fixed them, every
+    // thread below nw places its own slab.
+    if (a.fixed_allowed) {   // uniform
+        const unsigned bits = a.fixed_allowed;
+        if ((int)threadIdx.x < nw && ((bits >> threadIdx.x) & 1u))
+            allowed[__popc(bits & ((1u << threadIdx.x) - 1u))] = threadIdx.x;
+        if (threadIdx.x == 0) {
+            n_allowed = __popc(bits);
+            allowed_bits = bits;
+        }
+        __syncthreads();
+    } else {
+        // best score of every dz slab (16 threads per slab, then a shuffle
+        // reduction)
+        {
+            const int d = threadIdx.x >> 4, part = threadIdx.x & 15;
+            float m = __builtin_huge_valf();
+            if (d < nw && scored(d))
+                for (int c = d * nw2 + part; c < (d + 1) * nw2; c += 16) m = fminf(m, key[c]);
 #pragma unroll
-        for (int off = 8; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off, 16));
-        if (d < nw && part == 0) dzmin[d] = m;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        bool ok[16];
-        for (int d = 0; d < nw; ++d) ok[d] = a.max_slabs <= 0 || a.max_slabs >= nw;
-        int cnt = 0;
-        for (int d = 0; d < nw; ++d) cnt += ok[d];
-        for (int l = 0; l < a.n_lattice_dz && cnt < nw; ++l) {
-            const int d = a.lattice_dz[l] + hw;
-            if (!ok[d]) {
-                ok[d] = true;
+            for (int off = 8; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off, 16));
+            if (d < nw && part == 0) dzmin[d] = m;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            bool ok[16];
+            for (int d = 0; d < nw; ++d) ok[d] = a.max_slabs <= 0 || a.max_slabs >= nw;
+            int cnt = 0;
+            for (int d = 0; d < nw; ++d) cnt += ok[d];
+            for (int l = 0; l < a.n_lattice_dz && cnt < nw; ++l) {
+                const int d = a.lattice_dz[l] + hw;
+                if (!ok[d]) {
+                    ok[d] = true;
+                    ++cnt;
+                }
+            }
+            while (cnt < a.max_slabs && cnt < nw) {
+                int best = -1;
+                for (int d = 0; d < nw; ++d)
+                    if (!ok[d] && (best < 0 || dzmin[d] < dzmin[best])) best = d;
+                ok[best] = true;
                 ++cnt;
             }
-        }
-        while (cnt < a.max_slabs && cnt < nw) {
-            int best = -1;
+            int m = 0;
+            unsigned bits = 0;
             for (int d = 0; d < nw; ++d)
-                if (!ok[d] && (best < 0 || dzmin[d] < dzmin[best])) best = d;
-            ok[best] = true;
-            ++cnt;
+                if (ok[d]) {
+                    allowed[m++] = d;
+                    bits |= 1u << d;
+                }
+            n_allowed = m;
+            allowed_bits = bits;
         }
-        int m = 0;
-        unsigned bits = 0;
-        for (int d = 0; d < nw; ++d)
-            if (ok[d]) {
-                allowed[m++] = d;
-                bits |= 1u << d;
-            }
-        n_allowed = m;
-        allowed_bits = bits;
+        __syncthreads();
     }
-    __syncthreads();
     if (a.margin <= 0.f) {
         // The flag does not depend on the order statistics here, so the kept
         // set - the topk smallest (score, offset index) pairs of the allowed
@@ -1673,6 +1688,18 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     sa.n_lattice_dz = n_lattice_dz;
     for (int l = 0; l < 3; ++l) sa.lattice_dz[l] = lattice_dz[l];
     sa.score_bits = score_bits;
+    // the allowed slabs the kernel would choose, where the scores cannot
+    // change them: every slab, or the lattice's when they reach max_slabs
+    {
+        unsigned lat_bits = 0;
+        for (int l = 0; l < n_lattice_dz; ++l) lat_bits |= 1u << (lattice_dz[l] + hw);
+        if (opts->max_slabs <= 0 || opts->max_slabs >= nw)
+            sa.fixed_allowed = (1u << nw) - 1u;
+        else if (n_lattice_dz >= opts->max_slabs)
+            sa.fixed_allowed = lat_bits;
+        else
+            sa.fixed_allowed = 0u;
+    }
     if (ncand > kSelThreads * kSelPerThread) return PCT_ERR_UNSUPPORTED;
     // largest compacted candidate count: every slab, or the allowed ones
     const int slabs = (opts->max_slabs <= 0 || opts->max_slabs >= nw)
