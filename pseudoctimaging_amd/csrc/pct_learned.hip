@@ -264,6 +264,7 @@ struct EmbedWinArgs {
 // the values equal embed_kernel's bit for bit.
 __global__ __launch_bounds__(kEmbWaves * 64) void embed_windows_kernel(EmbedWinArgs a) {
     __shared__ float rows[kEmbWaves][3 * 18 * kWinRow];
+    __shared__ __align__(16) float sbias[144];   // b1[64], b2[64], b3[16]
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, n = lane & 15;
     v4s w1[4][2], w2[4][4], w3[4];
@@ -277,16 +278,13 @@ __global__ __launch_bounds__(kEmbWaves * 64) void embed_windows_kernel(EmbedWinA
         for (int s = 0; s < 4; ++s) w2[t][s] = a.frag[(8 + t * 4 + s) * 64 + lane];
 #pragma unroll
     for (int s = 0; s < 4; ++s) w3[s] = a.frag[(24 + s) * 64 + lane];
-    v4f b1[4], b2[4], b3;
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            b1[t][j] = a.bias[16 * t + 4 * g + j];
-            b2[t][j] = a.bias[64 + 16 * t + 4 * g + j];
-        }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) b3[j] = a.bias[128 + 4 * g + j];
+    // the biases stay in LDS and are read per row (128-bit reads), which
+    // keeps 36 VGPRs free for more waves per SIMD
+    for (int i = threadIdx.x; i < 144; i += kEmbWaves * 64) sbias[i] = a.bias[i];
+    __syncthreads();
+    auto bias4 = [&](int i) {   // opaque: read where used, not hoisted out of the loop
+        return *reinterpret_cast<const v4f*>(&sbias[opaque(i)]);
+    };
 
     float* __restrict__ R = rows[wv];
     const long long vsize = (long long)a.nx * a.ny * a.nz;
@@ -371,7 +369,7 @@ __global__ __launch_bounds__(kEmbWaves * 64) void embed_windows_kernel(EmbedWinA
                 v4s h1b[4];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
-                    v4f acc = b1[t];
+                    v4f acc = bias4(16 * t + 4 * g);
                     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[t][0], xb0, acc, 0, 0, 0);
                     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[t][1], xb1, acc, 0, 0, 0);
                     h1b[t] = relu_bf16(acc);
@@ -379,14 +377,14 @@ __global__ __launch_bounds__(kEmbWaves * 64) void embed_windows_kernel(EmbedWinA
                 v4s h2b[4];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
-                    v4f acc = b2[t];
+                    v4f acc = bias4(64 + 16 * t + 4 * g);
 #pragma unroll
                     for (int s = 0; s < 4; ++s)
                         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2[t][s], h1b[s], acc, 0,
                                                                           0, 0);
                     h2b[t] = relu_bf16(acc);
                 }
-                v4f e = b3;
+                v4f e = bias4(128 + 4 * g);
 #pragma unroll
                 for (int s = 0; s < 4; ++s)
                     e = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3[s], h2b[s], e, 0, 0, 0);
@@ -474,7 +472,62 @@ static int side_stream(int dev) {   // caller holds g_side.mu
     return PCT_OK;
 }
 
+// Per-device copy of the window embedding's z-plane list, uploaded again
+// only when the list changes: a copy from pageable memory makes the host
+// wait, which would stall every call.
+struct ZlistCache {
+    std::mutex mu;
+    int* d[64] = {};
+    size_t cap[64] = {};
+    std::vector<int> last[64];
+};
+static ZlistCache g_zlist;
+
+static int device_zlist(int dev, const std::vector<int>& zl, hipStream_t s, const int** out) {
+    if (dev < 0 || dev >= 64) return PCT_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> lk(g_zlist.mu);
+    if (g_zlist.cap[dev] < zl.size()) {
+        if (g_zlist.d[dev]) {
+            (void)hipStreamSynchronize(s);   // earlier calls may still read it
+            (void)hipFree(g_zlist.d[dev]);
+        }
+        g_zlist.d[dev] = nullptr;
+        g_zlist.cap[dev] = 0;
+        g_zlist.last[dev].clear();
+        if (hipMalloc(&g_zlist.d[dev], zl.size() * sizeof(int)) != hipSuccess) {
+            (void)hipGetLastError();
+            g_zlist.d[dev] = nullptr;
+            return PCT_ERR_NO_MEMORY;
+        }
+        g_zlist.cap[dev] = zl.size();
+    }
+    if (g_zlist.last[dev] != zl) {
+        // ordered on the caller's stream after the calls that read the old list
+        if (hipMemcpyAsync(g_zlist.d[dev], zl.data(), zl.size() * sizeof(int),
+                           hipMemcpyHostToDevice, s) != hipSuccess)
+            return PCT_ERR_HIP;
+        g_zlist.last[dev] = zl;
+    }
+    *out = g_zlist.d[dev];
+    return PCT_OK;
+}
+
 void release_learned_models() {
+    {
+        std::lock_guard<std::mutex> lk(g_zlist.mu);
+        for (int dev = 0; dev < 64; ++dev) {
+            if (!g_zlist.d[dev]) continue;
+            int cur = 0;
+            (void)hipGetDevice(&cur);
+            (void)hipSetDevice(dev);
+            (void)hipDeviceSynchronize();
+            (void)hipFree(g_zlist.d[dev]);
+            (void)hipSetDevice(cur);
+            g_zlist.d[dev] = nullptr;
+            g_zlist.cap[dev] = 0;
+            g_zlist.last[dev].clear();
+        }
+    }
     {
         std::lock_guard<std::mutex> lk(g_side.mu);
         for (int dev = 0; dev < 64; ++dev) {
@@ -1546,7 +1599,6 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     const size_t o_flag = off; off = align(off + (size_t)items);
     const size_t o_merged = off; off = align(off + sizeof(int) * (size_t)items * kMergedStride);
     const size_t o_scaled = off; off = align(off + sizeof(float) * (size_t)N * n);
-    const size_t o_zlist = off; off = align(off + sizeof(int) * (size_t)pb->nz);
     void* ws = nullptr;
     rc = ensure_workspace(dev, off, &ws);
     if (rc) return rc;
@@ -1634,11 +1686,15 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
                                  ((pb->ny + kBoxTY - 1) / kBoxTY);
         hipLaunchKernelGGL(kBoxXyz[hw - 1], dim3((unsigned)blocks), dim3(kBoxThreads), 0, ss,
                            d_label, boxL, pb->nx, pb->ny, pb->nz, n);
-        const unsigned bs = (unsigned)std::min<long long>((N * n + 255) / 256, 65536);
-        hipLaunchKernelGGL(scale_copy, dim3(bs), dim3(256), 0, ss, d_atlas, scaled, N * n,
-                           pa.t.scale);
         if (hipGetLastError() != hipSuccess || hipEventRecord(g_side.join[dev], ss) != hipSuccess)
             return PCT_ERR_HIP;
+    }
+    {
+        // the scaled atlas copy (HBM-bound) first on the caller's stream, beside the window sums
+        const unsigned bs = (unsigned)std::min<long long>((N * n + 255) / 256, 65536);
+        hipLaunchKernelGGL(scale_copy, dim3(bs), dim3(256), 0, s, d_atlas, scaled, N * n,
+                           pa.t.scale);
+        if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
     }
 
     // 1. embeddings of the target and of every atlas MRI
@@ -1650,14 +1706,8 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
         if (!rc) rc = launch_embed(params, model, d_atlas, pb->nx, pb->ny, pb->nz, n, embA, s);
     } else {
         rc = launch_embed_windows(params, model, d_img, 1, 0, pb, g, t_oy, t_wx, sh.bz, embT, s);
-        int* d_zlist = nullptr;
-        if (!rc && !zlist.empty()) {
-            // pageable source: the copy has taken the data when the call returns
-            d_zlist = reinterpret_cast<int*>(w + o_zlist);
-            if (hipMemcpyAsync(d_zlist, zlist.data(), zlist.size() * sizeof(int),
-                               hipMemcpyHostToDevice, s) != hipSuccess)
-                rc = PCT_ERR_HIP;
-        }
+        const int* d_zlist = nullptr;
+        if (!rc && !zlist.empty()) rc = device_zlist(dev, zlist, s, &d_zlist);
         if (!rc)
             rc = launch_embed_windows(params, model, d_atlas, n, hw, pb, g, t_oy, t_wx, sh.bz,
                                       embA, s, d_zlist, (int)zlist.size());
