@@ -265,22 +265,20 @@ struct EmbedWinArgs {
 __global__ __launch_bounds__(kEmbWaves * 64) void embed_windows_kernel(EmbedWinArgs a) {
     __shared__ float rows[kEmbWaves][3 * 18 * kWinRow];
     __shared__ __align__(16) float sbias[144];   // b1[64], b2[64], b3[16]
+    __shared__ v4s sw2[16][64];                  // layer-2 weight fragments (t, s) by lane
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, n = lane & 15;
-    v4s w1[4][2], w2[4][4], w3[4];
+    v4s w1[4][2], w3[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
         for (int s = 0; s < 2; ++s) w1[t][s] = a.frag[(t * 2 + s) * 64 + lane];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) w2[t][s] = a.frag[(8 + t * 4 + s) * 64 + lane];
-#pragma unroll
     for (int s = 0; s < 4; ++s) w3[s] = a.frag[(24 + s) * 64 + lane];
     // the biases stay in LDS and are read per row (128-bit reads), which
     // keeps 36 VGPRs free for more waves per SIMD
     for (int i = threadIdx.x; i < 144; i += kEmbWaves * 64) sbias[i] = a.bias[i];
+    for (int i = threadIdx.x; i < 16 * 64; i += kEmbWaves * 64) sw2[i >> 6][i & 63] = a.frag[8 * 64 + i];
     __syncthreads();
     auto bias4 = [&](int i) {   // opaque: read where used, not hoisted out of the loop
         return *reinterpret_cast<const v4f*>(&sbias[opaque(i)]);
@@ -380,8 +378,8 @@ __global__ __launch_bounds__(kEmbWaves * 64) void embed_windows_kernel(EmbedWinA
                     v4f acc = bias4(64 + 16 * t + 4 * g);
 #pragma unroll
                     for (int s = 0; s < 4; ++s)
-                        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2[t][s], h1b[s], acc, 0,
-                                                                          0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                            sw2[opaque(t * 4 + s)][lane], h1b[s], acc, 0, 0, 0);
                     h2b[t] = relu_bf16(acc);
                 }
                 v4f e = bias4(128 + 4 * g);
