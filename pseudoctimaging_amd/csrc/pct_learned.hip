@@ -10,6 +10,8 @@
 //                 16x16 output tile of one layer is, lane for lane, the B
 //                 operand of the next layer, so no data moves between layers;
 //                 the 28 weight fragments stay in VGPRs.
+//  embed_windows_kernel  the same chain over only the voxels the selection
+//                 reads; biases and layer-2 weights in LDS (more waves).
 //  box_xyz        NH-wide box sums of the atlas CT in one pass (per plane an
 //                 LDS pass over y and x, then running sums over z).
 //  select_kernel  per (tile, atlas): embedding distance of every window
@@ -19,6 +21,8 @@
 //                 bitonic sort where a positive margin needs the order
 //                 statistics), a low score gap flags the item for the full
 //                 search.
+//  merge_offsets_kernel  per item, the offsets to evaluate in the pruned
+//                 kernel's order (kept, then samples, per dz plane).
 //  pruned_kernel  per (tile, atlas): the kept offsets and a 3x3x3 lattice of
 //                 sample offsets, weighted exactly with the same LDS slabs and
 //                 separable box sums as the plain tiled kernel; the other
