@@ -15,7 +15,7 @@ Fixtures
   small_uniform.npz   random-mode data (10*U[0,1)), 14x15x13, n=2, NH=3,
                       sigma 0.33, beta 1: a mix of NaN and finite voxels, so the
                       NaN rule and the median fill are exercised.
-  cfg1.json           64x64x32, n=2, NH=7: input and
+  cfg1.json           64x64x32, n=2, NH=7 (the CPU-sized case): input and
                       output sha256 plus 2048 sampled voxel values.
   cfg2_sample.npz     222x222x112, n=5, NH=9 and
   cfg3_sample.npz     222x222x112, n=10, NH=11: input sha256, sampled voxel
