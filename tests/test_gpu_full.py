@@ -333,7 +333,9 @@ def test_autotune_keeps_results(gpu):
     compare(out.cpu().numpy(), ref)
 
 
-def test_cli_end_to_end(gpu, tmp_path):
+@pytest.mark.parametrize("choice", [pct.CHOICE_SM, pct.CHOICE_GM, pct.CHOICE_DIRECT])
+def test_cli_end_to_end(gpu, tmp_path, choice):
+    # EXEC_CHOICE 2 (SM) and 0 (GM) run the tiled kernel, 5 the direct one
     nx, ny, nz, n, nh = 26, 34, 24, 2, 5
     img, amri, act = pct.synth_tissue(nx, ny, nz, n, 2, False, 33)
     paths = [str(tmp_path / f) for f in ("atlas.txt", "label.txt", "im.txt")]
@@ -345,8 +347,8 @@ def test_cli_end_to_end(gpu, tmp_path):
         f.write("0\n" + "\n".join(paths) + "\n")
     cli = os.path.join(os.path.dirname(_lib.LIB_PATH), "pct_cli")
     res = str(tmp_path / "result")
-    r = subprocess.run([cli, cfg, res, "2", str(nh), "2.5", "1", "0"], capture_output=True,
-                       text=True, timeout=120)
+    r = subprocess.run([cli, cfg, res, str(choice), str(nh), "2.5", "1", "0"],
+                       capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
     hu, dims = pct.matrixio_read(res + ".mat")
@@ -358,7 +360,8 @@ def test_cli_end_to_end(gpu, tmp_path):
     fused = oracle.fuse(d["img"], d["atlas"], lab, nh, 2.5, 1.0)
     ref_hu = oracle.postprocess(fused.copy(), nh)
     got = hu.reshape(nz, nx, ny)
-    gpu_fused = pct.multi_patch_based_segmentation_gpu(d["img"], d["atlas"], lab, nh, 2.5, 1.0)
+    gpu_fused = pct.multi_patch_based_segmentation_gpu(d["img"], d["atlas"], lab, nh, 2.5, 1.0,
+                                                       choice)
     compare(gpu_fused, fused)
     _final_hu_agrees(gpu_fused, fused, ref_hu, nh)
     assert np.array_equal(got, pct.postprocess(gpu_fused.copy(), nh))
