@@ -1810,11 +1810,14 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
             return PCT_ERR_HIP;
     }
 
-    // 5. full exact search for the flagged items
-    TiledArgs fa = pa.t;
-    fa.only = sa.flag;
-    rc = launch_tiled_items(fa, shape_id, hw, s);
-    if (rc) return rc;
+    // 5. full exact search for the flagged items (with margin 0 the
+    // selection flags none, and the launch is skipped)
+    if (opts->margin != 0.f) {
+        TiledArgs fa = pa.t;
+        fa.only = sa.flag;
+        rc = launch_tiled_items(fa, shape_id, hw, s);
+        if (rc) return rc;
+    }
 
     // 6. sum over atlases, NaN rule, zero border
     rc = launch_finalize(pb, g1, part, d_out, s);
