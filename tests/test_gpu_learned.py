@@ -212,6 +212,46 @@ def test_pruned_full_fallback_is_bit_exact(gpu):
     assert np.array_equal(out.cpu().numpy().view(np.uint32), ref.cpu().numpy().view(np.uint32))
 
 
+@pytest.mark.parametrize("shape", [(13, 20, 15), (14, 13, 40), (20, 14, 13)])
+def test_pruned_thin_interior(gpu, shape):
+    # NH = 11 (r = 6) on volumes whose interior is 1-3 voxels thick along one
+    # axis: probe clusters, window planes and window sums sit against the
+    # faces.  The full fallback equals the plain kernel bit for bit; the
+    # pruned fusion stays finite inside, zero on the border, and close to
+    # the exact fusion.
+    import torch
+    nh, r = 11, 6
+    img, amri, lab = _fusion_case(shape, 1, nh, 25)
+    p = learned.mlp_init(6, *learned.intensity_stats(img))
+    dimg, dam, dlab = _dev(img), _dev(amri), _dev(lab)
+    print(f"case thin interior {shape}", flush=True)
+    ref = torch.zeros(img.shape, device="cuda:0")
+    try:
+        pct.force_tile_config(13, 1)
+        pct.fuse_device(dimg, dam, dlab, ref, nh, 2.5, 1.0)
+    finally:
+        pct.force_tile_config(-1, 0)
+    full = torch.full(img.shape, -1.0, device="cuda:0")
+    frac = learned.fuse_pruned_device(dimg, dam, dlab, full, nh, 2.5, 1.0, p, topk=32,
+                                      margin=-1.0, shape_id=13)
+    assert frac == 1.0
+    out = torch.full(img.shape, -1.0, device="cuda:0")
+    frac0 = learned.fuse_pruned_device(dimg, dam, dlab, out, nh, 2.5, 1.0, p, topk=32,
+                                       margin=0.0)
+    torch.cuda.synchronize()
+    assert frac0 == 0.0
+    e, f, o = ref.cpu().numpy(), full.cpu().numpy(), out.cpu().numpy()
+    assert np.array_equal(f.view(np.uint32), e.view(np.uint32))
+    nz, nx, ny = shape
+    inner = (slice(r, nz - r), slice(r, nx - r), slice(r, ny - r))
+    assert np.isfinite(o[inner]).all()
+    border = o.copy()
+    border[inner] = 0.0
+    assert not border.any()
+    mae, _ = _hu_mae(o[inner], e[inner])
+    assert mae < 10.0, mae
+
+
 @pytest.mark.parametrize("case", [((30, 40, 44), 2, 11, 32, 4), ((26, 36, 30), 3, 9, 16, 0),
                                   ((24, 30, 28), 2, 7, 64, 2), ((20, 22, 26), 1, 5, 7, 4)])
 def test_radix_selection_equals_sorted_selection(gpu, case):
