@@ -276,18 +276,20 @@ def test_radix_selection_equals_sorted_selection(gpu, case):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-@pytest.mark.parametrize("shape", [(30, 40, 44), (16, 17, 50), (40, 70, 21)])
-def test_window_embedding_equals_full(gpu, shape):
-    # The pruned path embeds only the atlas voxels the selection reads; with
-    # every voxel embedded instead, the result is bit-identical.
+@pytest.mark.parametrize("shape,nh", [((30, 40, 44), 11), ((16, 17, 50), 11), ((40, 70, 21), 11),
+                                      ((24, 30, 28), 5), ((20, 22, 26), 7), ((34, 36, 40), 15)])
+def test_window_embedding_equals_full(gpu, shape, nh):
+    # The pruned path embeds only the atlas voxels the selection reads (the
+    # window planes of the scored dz); with every voxel embedded instead, the
+    # result is bit-identical.
     import torch
-    img, amri, lab = _fusion_case(shape, 2, 11, 23)
+    img, amri, lab = _fusion_case(shape, 2, nh, 23)
     p = _random_params(9, *learned.intensity_stats(img))
     dimg, dam, dlab = _dev(img), _dev(amri), _dev(lab)
     outs = []
     for full in (False, True):
         out = torch.full(img.shape, -9.0, device="cuda:0")
-        learned.fuse_pruned_device(dimg, dam, dlab, out, 11, 2.5, 1.0, p, topk=16, samples=27,
+        learned.fuse_pruned_device(dimg, dam, dlab, out, nh, 2.5, 1.0, p, topk=16, samples=27,
                                    margin=0.05, embed_all=full)
         torch.cuda.synchronize()
         outs.append(out.cpu().numpy())
