@@ -962,6 +962,12 @@ __device__ __forceinline__ void select_topk_radix(
 #pragma unroll
     for (int q = 0; q < kSelPerThread; ++q) {
         if (q * kSelThreads >= ncand) break;   // uniform over the workgroup
+        {
+            // a round without allowed slabs keeps nothing: skip its barriers
+            const int lo = q * kSelThreads / nw2;
+            const int hi = (min(ncand, (q + 1) * kSelThreads) - 1) / nw2;
+            if (((allowed_bits >> lo) & ((2u << (hi - lo)) - 1u)) == 0u) continue;
+        }
         const unsigned long long bal = __ballot(keep[q]);
         if (lane == 0) wcnt[wv] = __popcll(bal);
         __syncthreads();
