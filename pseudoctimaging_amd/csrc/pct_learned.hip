@@ -443,8 +443,8 @@ struct ModelCache {
 static ModelCache g_model;
 
 // Second stream of the pruned call, per device, created on first use at the
-// default priority: the CT window sums and the scaled atlas copy depend on
-// neither the embeddings nor the selection and run beside them.  The fork
+// default priority: the CT window sums depend on neither the embeddings nor
+// the selection and run beside them.  The fork
 // and join events are recorded and waited on under `mu`, so calls from
 // several host threads cannot re-record one between its record and its wait.
 struct SideStreams {
@@ -810,14 +810,6 @@ using BoxFn = void (*)(const float*, float*, int, int, int, int);
 static const BoxFn kBoxXyz[kMaxTiledHalfWindow] = {&box_xyz<1>, &box_xyz<2>, &box_xyz<3>,
                                                    &box_xyz<4>, &box_xyz<5>, &box_xyz<6>,
                                                    &box_xyz<7>};
-
-// out = in * scale (the atlas MRI as the tiled kernels stage it).
-__global__ void scale_copy(const float* __restrict__ in, float* __restrict__ out, long long n,
-                           float scale) {
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x)
-        out[i] = in[i] * scale;
-}
 
 // ---------------------------------------------------------------------------
 // candidate selection
@@ -1280,7 +1272,6 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
 
 struct PrunedArgs {
     TiledArgs t;                    // problem, tiles (group = 1), workspace partials
-    const float* atlas_scaled;      // [n][N] atlas MRI times t.scale
     const int* sel;                 // [items][topk]
     const unsigned char* flag;      // [items]
     const float* boxL;              // [n][N] window sums of the CT
@@ -1463,14 +1454,14 @@ __global__ __launch_bounds__(NW * 64) void pruned_kernel(PrunedArgs pa) {
     const int ys = y0 - 1 - HW;
     DirectPlan<C, NW> dp;
     dp.init(xs, ys, nx, ny, wv, lane);
-    const float* __restrict__ As = pa.atlas_scaled + (long long)at * args.N;
+    const float* __restrict__ A = args.atlas + (long long)at * args.N;
     const float* __restrict__ L = args.label + (long long)at * args.N;
     const int m = n_offs;
     int k = 0;
     while (k < m) {
         const int dz = odz[k];
         __syncthreads();
-        stage_slab_direct<C, NW>(lds, As, L, z0 + dz, nz, plane, wv, dp);
+        stage_slab_direct<C, NW>(lds, A, L, args.scale, z0 + dz, nz, plane, wv, dp);
         for (; k < m && odz[k] == dz && !is_sample[k]; ++k)
             pruned_step<C, BX, BZ, false>(lds, lane_base + odel[k], tv, P, S, Le, Ws);
         for (; k < m && odz[k] == dz; ++k)
@@ -1606,7 +1597,6 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     const size_t o_sel = off; off = align(off + sizeof(int) * (size_t)items * topk);
     const size_t o_flag = off; off = align(off + (size_t)items);
     const size_t o_merged = off; off = align(off + sizeof(int) * (size_t)items * kMergedStride);
-    const size_t o_scaled = off; off = align(off + sizeof(float) * (size_t)N * n);
     void* ws = nullptr;
     rc = ensure_workspace(dev, off, &ws);
     if (rc) return rc;
@@ -1666,10 +1656,10 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
             if (need[(size_t)z]) zlist.push_back(z);
     }
 
-    // the pruned kernel's arguments (problem, tiles, partials) and the two
-    // inputs it reads besides the selection: the window sums of every atlas
-    // CT and the atlas MRI pre-scaled for the direct LDS loads.  Those two
-    // run on the second stream, beside steps 1 and 3.
+    // the pruned kernel's arguments (problem, tiles, partials) and the input
+    // it reads besides the selection and the volumes: the window sums of every
+    // atlas CT, on the second stream beside steps 1 and 3.  The pruned kernel
+    // copies the unscaled atlas MRI into LDS and scales it there.
     float* part = reinterpret_cast<float*>(w + o_part);
     PrunedArgs pa;
     std::memset(&pa, 0, sizeof(pa));
@@ -1679,7 +1669,6 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     g1.n_items = items;
     pa.t = make_tiled_args(pb, g1, d_img, d_atlas, d_label, part);
     float* boxL = reinterpret_cast<float*>(w + o_boxL);
-    float* scaled = reinterpret_cast<float*>(w + o_scaled);
     std::unique_lock<std::mutex> side_lk(g_side.mu);   // until the join is enqueued on s
     rc = side_stream(dev);
     if (rc) return rc;
@@ -1696,13 +1685,6 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
                            d_label, boxL, pb->nx, pb->ny, pb->nz, n);
         if (hipGetLastError() != hipSuccess || hipEventRecord(g_side.join[dev], ss) != hipSuccess)
             return PCT_ERR_HIP;
-    }
-    {
-        // the scaled atlas copy (HBM-bound) first on the caller's stream, beside the window sums
-        const unsigned bs = (unsigned)std::min<long long>((N * n + 255) / 256, 65536);
-        hipLaunchKernelGGL(scale_copy, dim3(bs), dim3(256), 0, s, d_atlas, scaled, N * n,
-                           pa.t.scale);
-        if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
     }
 
     // 1. embeddings of the target and of every atlas MRI
@@ -1782,7 +1764,6 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     // after the second stream's work
     if (hipStreamWaitEvent(s, g_side.join[dev], 0) != hipSuccess) return PCT_ERR_HIP;
     side_lk.unlock();
-    pa.atlas_scaled = scaled;
     pa.sel = sa.sel;
     pa.flag = sa.flag;
     pa.boxL = boxL;

@@ -256,14 +256,16 @@ struct DirectPlan {
     }
 };
 
-// Copies the slab of stage_slab (the MRI already scaled, `As`); positions
-// outside the volume read zeros.  Ends with every wave's loads complete and a
-// workgroup barrier.
+// Copies the slab of stage_slab from the unscaled atlas MRI; positions
+// outside the volume read zeros.  Once its own loads have landed, each lane
+// multiplies the MRI positions they wrote by `scale`: the product stage_slab
+// forms before its LDS store (zeros stay zero), so both stagings hold the same
+// bits.  Ends with a workgroup barrier.
 template <class C, int NW>
 __device__ __forceinline__ void stage_slab_direct(float* __restrict__ lds,
-                                                  const float* __restrict__ As,
-                                                  const float* __restrict__ L, int zdz, int nz,
-                                                  long long plane, int wv,
+                                                  const float* __restrict__ A,
+                                                  const float* __restrict__ L, float scale,
+                                                  int zdz, int nz, long long plane, int wv,
                                                   const DirectPlan<C, NW>& dp) {
     typedef __attribute__((address_space(3))) void* lds_ptr;
     constexpr int NP = C::APLANES + C::LPLANES;
@@ -273,7 +275,7 @@ __device__ __forceinline__ void stage_slab_direct(float* __restrict__ lds,
         const bool is_mri = pl < C::APLANES;
         const int zz = is_mri ? zdz - 1 + pl : zdz + (pl - C::APLANES);
         const bool zok = zz >= 0 && zz < nz;
-        const float* base = (is_mri ? As : L) + (zok ? zz : 0) * plane;
+        const float* base = (is_mri ? A : L) + (zok ? zz : 0) * plane;
 #pragma unroll
         for (int k = 0; k < P::JN; ++k) {
             const int j = wv + k * NW;
@@ -286,6 +288,17 @@ __device__ __forceinline__ void stage_slab_direct(float* __restrict__ lds,
         }
     }
     __builtin_amdgcn_s_waitcnt(0);
+    __asm__ volatile("" ::: "memory");
+    // lane l of load (plane pl, chunk j) wrote LDS element pl * PS + j * 64 + l
+    const int lane = (int)(threadIdx.x & 63u);
+#pragma unroll
+    for (int pl = 0; pl < C::APLANES; ++pl)
+#pragma unroll
+        for (int k = 0; k < P::JN; ++k)
+            if (dp.off[k] >= -1) {
+                float* q = lds + pl * C::PS + (wv + k * NW) * 64 + lane;
+                *q = *q * scale;
+            }
     __syncthreads();
 }
 
