@@ -119,11 +119,35 @@ def _dev_ptr(t, name: str):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def check_volume_set(img, atlas, atlas_label, out=None, stream=None) -> None:
+    """Checks, before anything is launched, that device tensors form one
+    problem: img (nz, nx, ny), atlas and atlas_label (n, nz, nx, ny), out
+    shaped as img, every tensor on one device and the stream on it too.  The
+    kernels take their sizes from img and atlas alone, so a smaller label or
+    out tensor would otherwise be read or written past its end."""
+    if getattr(img, "ndim", None) != 3:
+        raise ValueError("img must be (nz, nx, ny)")
+    if getattr(atlas, "ndim", None) != 4 or tuple(atlas.shape[1:]) != tuple(img.shape):
+        raise ValueError("atlas must be (n, nz, nx, ny) with (nz, nx, ny) = img.shape")
+    if atlas_label is not None and tuple(atlas_label.shape) != tuple(atlas.shape):
+        raise ValueError("atlas_label must have the shape of atlas")
+    if out is not None and tuple(out.shape) != tuple(img.shape):
+        raise ValueError("out must have the shape of img")
+    tensors = [t for t in (img, atlas, atlas_label, out) if t is not None]
+    devices = {str(getattr(t, "device", "host")) for t in tensors}
+    if len(devices) != 1:
+        raise ValueError(f"tensors on different devices: {sorted(devices)}")
+    if stream is not None and getattr(stream, "device", None) is not None \
+            and str(stream.device) != str(img.device):
+        raise ValueError(f"stream on {stream.device}, tensors on {img.device}")
+
+
 def fuse_device(img, atlas, atlas_label, out, nh: int, sigma: float = 1.0, beta: float = 1.0,
                 choice: int = CHOICE_SM, stream=None) -> None:
     """Fusion on tensors already resident in HBM (torch only carries memory
     and the stream).  Enqueued on `stream` (default: torch's current one)."""
     import torch
+    check_volume_set(img, atlas, atlas_label, out, stream)
     if stream is None:
         stream = torch.cuda.current_stream(img.device)
     pb = problem(img.shape, atlas.shape[0], nh, sigma, beta)
@@ -152,6 +176,7 @@ def tile_config(shape, n_atlas: int, nh: int) -> dict:
 def autotune(img, atlas, atlas_label, out, nh: int, sigma: float = 1.0, beta: float = 1.0,
              reps: int = 2, stream=None) -> dict:
     import torch
+    check_volume_set(img, atlas, atlas_label, out, stream)
     if stream is None:
         stream = torch.cuda.current_stream(img.device)
     pb = problem(img.shape, atlas.shape[0], nh, sigma, beta)
@@ -172,6 +197,12 @@ def force_tile_config(config_id: int, atlas_group: int = 0) -> None:
 def _subjects(subjects):
     """ctypes array of pct_subject from dicts with device tensors img, atlas,
     label (normalised CT), out and the window parameters nh, sigma, beta."""
+    devices = set()
+    for s in subjects:
+        check_volume_set(s["img"], s["atlas"], s["label"], s["out"])
+        devices.add(str(s["img"].device))
+    if len(devices) > 1:
+        raise ValueError(f"subjects on different devices: {sorted(devices)}")
     arr = (_lib.Subject * len(subjects))()
     for i, s in enumerate(subjects):
         arr[i].pb = problem(s["img"].shape, s["atlas"].shape[0], s["nh"], s.get("sigma", 1.0),
@@ -187,18 +218,20 @@ def fuse_batch(subjects, stream=None) -> None:
     """Fuses every subject (list of dicts, see _subjects) in one graph-captured
     persistent launch; asynchronous on `stream`."""
     import torch
-    if stream is None:
-        stream = torch.cuda.current_stream()
     arr = _subjects(subjects)
+    if stream is None:
+        stream = torch.cuda.current_stream(subjects[0]["img"].device)
+    check_volume_set(subjects[0]["img"], subjects[0]["atlas"], None, None, stream)
     check(lib().pct_fuse_batch_dev(arr, len(subjects), ctypes.c_void_p(stream.cuda_stream)),
           "pct_fuse_batch_dev")
 
 
 def autotune_batch(subjects, reps: int = 1, stream=None) -> dict:
     import torch
-    if stream is None:
-        stream = torch.cuda.current_stream()
     arr = _subjects(subjects)
+    if stream is None:
+        stream = torch.cuda.current_stream(subjects[0]["img"].device)
+    check_volume_set(subjects[0]["img"], subjects[0]["atlas"], None, None, stream)
     cfg = TileConfig()
     check(lib().pct_autotune_batch(arr, len(subjects), int(reps),
                                    ctypes.c_void_p(stream.cuda_stream), byref(cfg)),

@@ -387,11 +387,9 @@ bool read_sidecar(const char* path, float** data, int dims4[4]) {
     SidecarHeader h;
     bool ok = std::fread(&h, sizeof(h), 1, f) == 1 &&
               std::memcmp(h.magic, kSidecarMagic, sizeof(kSidecarMagic)) == 0 &&
-              h.text_bytes == bytes && h.text_mtime_ns == mtime && h.num > 0 && h.rows > 0 &&
-              h.cols > 0 && h.depth > 0 && h.num <= INT_MAX && h.rows <= INT_MAX &&
-              h.cols <= INT_MAX && h.depth <= INT_MAX;
-    const long long size = ok ? h.num * h.rows * h.cols * h.depth : 0;
-    ok = ok && size <= (1LL << 34);
+              h.text_bytes == bytes && h.text_mtime_ns == mtime;
+    long long per_volume = 0, size = 0;
+    ok = ok && checked_volume_elements(h.num, h.rows, h.cols, h.depth, &per_volume, &size);
     float* v = ok ? static_cast<float*>(std::malloc(sizeof(float) * (size_t)size)) : nullptr;
     ok = ok && v && std::fread(v, sizeof(float), (size_t)size, f) == (size_t)size &&
          std::fgetc(f) == EOF;
@@ -425,10 +423,11 @@ int read_text(const char* path, float** data, int dims4[4]) {
     }
     const long long num = hdr[0], rows = hdr[1], cols = hdr[2], depth = hdr[3];
     if (num <= 0 || rows <= 0 || cols <= 0 || depth <= 0) return PCT_ERR_FORMAT;
-    if (num > INT_MAX || rows > INT_MAX || cols > INT_MAX || depth > INT_MAX)
+    // every product is checked before the allocation: a wrapped count would
+    // give a short buffer that later loops overrun
+    long long per_volume = 0, size = 0;
+    if (!checked_volume_elements(num, rows, cols, depth, &per_volume, &size))
         return PCT_ERR_TOO_LARGE;
-    const long long size = num * rows * cols * depth;
-    if (size > (1LL << 34)) return PCT_ERR_TOO_LARGE;
     float* v = static_cast<float*>(std::malloc(sizeof(float) * (size_t)size));
     if (!v) return PCT_ERR_NO_MEMORY;
     // values separated by blanks/newlines, in memory order (cpp:273-285)

@@ -2,6 +2,7 @@
 // libpct.so.  Not part of the C ABI.
 #pragma once
 
+#include <climits>
 #include <cstddef>
 #include <cstdint>
 
@@ -31,6 +32,27 @@ struct TileGeometry {
 
 // Problem validation shared by every entry point.
 int validate_problem(const pct_problem* pb);
+
+// Largest element count a reader accepts for all of a file's volumes together.
+constexpr long long kMaxFileElements = 1LL << 34;
+
+// Element counts of `num` volumes of rows x cols x depth read from a file,
+// with every product checked for overflow before it is used: false when any
+// dimension is below 1, one volume holds INT32_MAX elements or more, or the
+// total exceeds kMaxFileElements.
+inline bool checked_volume_elements(long long num, long long rows, long long cols,
+                                    long long depth, long long* per_volume,
+                                    long long* total) {
+    if (num < 1 || rows < 1 || cols < 1 || depth < 1) return false;
+    long long plane = 0, vol = 0, all = 0;
+    if (__builtin_mul_overflow(rows, cols, &plane) || __builtin_mul_overflow(plane, depth, &vol) ||
+        vol >= (long long)INT32_MAX || __builtin_mul_overflow(vol, num, &all) ||
+        all > kMaxFileElements)
+        return false;
+    *per_volume = vol;
+    *total = all;
+    return true;
+}
 
 // Host-side helpers implemented in pct_host.cpp.
 uint64_t splitmix64(uint64_t& state);

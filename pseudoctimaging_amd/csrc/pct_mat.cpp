@@ -18,6 +18,7 @@
 #include <zlib.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -27,7 +28,10 @@
 #include <string>
 #include <vector>
 
-#include "pct/pct.h"
+#include "pct_internal.h"
+
+using pct::checked_volume_elements;
+using pct::kMaxFileElements;
 
 namespace {
 
@@ -129,9 +133,12 @@ struct MatArray {
 
 bool parse_matrix(const Element& m, Store* store, MatArray& a);
 
+// Element count of an array, -1 when the product of its dimensions
+// overflows or exceeds kMaxFileElements.
 long long count(const MatArray& a) {
     long long c = 1;
-    for (long long d : a.dims) c *= d;
+    for (long long d : a.dims)
+        if (__builtin_mul_overflow(c, d, &c) || c > kMaxFileElements) return -1;
     return c;
 }
 
@@ -175,6 +182,7 @@ bool parse_matrix(const Element& m, Store* store, MatArray& a) {
         return true;
     }
     if (a.cls < mxDOUBLE || a.cls > mxUINT64) return false;   // numeric classes only
+    if (count(a) < 0) return false;
     if (count(a) == 0) {
         a.real = Element{miSINGLE, nullptr, 0};
         return true;
@@ -283,13 +291,20 @@ extern "C" int pct_mat_read(const char* path, pct_volume_set* out) {
     if (img->dims.size() != 3 || atlas->values.empty() ||
         atlas->values.size() != label->values.size())
         return PCT_ERR_DIMS;
+    const long long num_ll = (long long)atlas->values.size();
+    long long per_volume = 0, total = 0;
+    // the atlases and labels hold 2 * num volumes; every product is checked
+    // before the allocations, so no count can wrap into a short buffer
+    if (!checked_volume_elements(2 * num_ll + 1, img->dims[0], img->dims[1], img->dims[2],
+                                 &per_volume, &total))
+        return img->dims[0] < 1 || img->dims[1] < 1 || img->dims[2] < 1 ? PCT_ERR_DIMS
+                                                                       : PCT_ERR_TOO_LARGE;
     const int nx = (int)img->dims[0], ny = (int)img->dims[1], nz = (int)img->dims[2];
-    const int num = (int)atlas->values.size();
-    if (nx < 1 || ny < 1 || nz < 1) return PCT_ERR_DIMS;
+    const int num = (int)num_ll;
     for (int k = 0; k < num; ++k)
         if (atlas->values[k].dims != img->dims || label->values[k].dims != img->dims)
             return PCT_ERR_DIMS;
-    const size_t n = (size_t)nx * ny * nz;
+    const size_t n = (size_t)per_volume;
     float* im = static_cast<float*>(std::malloc(sizeof(float) * n));
     float* at = static_cast<float*>(std::malloc(sizeof(float) * n * num));
     float* lb = static_cast<float*>(std::malloc(sizeof(float) * n * num));

@@ -20,7 +20,7 @@ from ctypes import byref
 import numpy as np
 
 from ._lib import (PCT_MLP_NPARAMS, PrunedOpts, TrainOpts, check, fptr, lib)
-from .api import _dev_ptr, _f32, problem
+from .api import _dev_ptr, _f32, check_volume_set, problem
 
 # parameter vector layout (input-major weights)
 OFF_W1, OFF_B1, OFF_W2, OFF_B2, OFF_W3, OFF_B3, OFF_MU, OFF_SD = (
@@ -90,6 +90,7 @@ def fuse_pruned_device(img, atlas, atlas_label, out, nh: int, sigma: float, beta
     the lattice samples' planes: at 222x222x112, n=10, NH=11 it measured
     faster than 4 (3.49 vs 3.80 ms) at the same error (1.90 HU MAE)."""
     params = _f32(params, "params")
+    check_volume_set(img, atlas, atlas_label, out, stream)
     s = _stream(img, stream)
     pb = problem(img.shape, atlas.shape[0], nh, sigma, beta)
     opts = PrunedOpts(topk=int(topk), samples=int(samples), margin=float(margin),
@@ -111,6 +112,7 @@ def mlp_loss_grad(img, atlas, nh: int, sigma: float, beta: float, params: np.nda
     uses the fusion divisor as teacher temperature."""
     params = _f32(params, "params")
     anc = np.ascontiguousarray(np.asarray(anchors, np.int32).reshape(-1, 4))
+    check_volume_set(img, atlas, None, None, stream)
     s = _stream(img, stream)
     pb = problem(img.shape, atlas.shape[0], nh, sigma, beta)
     loss = ctypes.c_double(0.0)
@@ -131,6 +133,7 @@ def mlp_train(img, atlas, nh: int, sigma: float, beta: float, params: np.ndarray
     teacher_t is the temperature of the SSD ordering the embedding learns
     (<= 0: the fusion divisor)."""
     p = np.array(_f32(params, "params"), copy=True)
+    check_volume_set(img, atlas, None, None, stream)
     s = _stream(img, stream)
     pb = problem(img.shape, atlas.shape[0], nh, sigma, beta)
     opts = TrainOpts(steps=int(steps), anchors=int(anchors), lr=float(lr), tau=float(tau),

@@ -117,3 +117,37 @@ def test_forced_configuration_round_trip():
         pct.force_tile_config(-1, 0)
     with pytest.raises(pct.PctError):
         pct.force_tile_config(999, 1)
+
+
+def test_device_wrappers_check_shapes_before_any_launch():
+    # the kernels size every buffer from img and atlas: a label or output
+    # tensor of another shape must be refused on the host, before a launch
+    import torch
+
+    from pseudoctimaging_amd import learned
+    img = torch.zeros(6, 7, 8)
+    atlas = torch.zeros(2, 6, 7, 8)
+    small_label = torch.zeros(1, 6, 7, 8)
+    small_out = torch.zeros(6, 7, 7)
+    calls = [
+        lambda: pct.fuse_device(img, atlas, small_label, torch.zeros(6, 7, 8), 3),
+        lambda: pct.fuse_device(img, atlas, atlas.clone(), small_out, 3),
+        lambda: pct.fuse_device(img, torch.zeros(2, 6, 7, 9), torch.zeros(2, 6, 7, 9),
+                                torch.zeros(6, 7, 8), 3),
+        lambda: pct.fuse_device(img[0], atlas, atlas.clone(), torch.zeros(6, 7, 8), 3),
+        lambda: pct.autotune(img, atlas, small_label, torch.zeros(6, 7, 8), 3),
+        lambda: pct.fuse_batch([{"img": img, "atlas": atlas, "label": small_label,
+                                 "out": torch.zeros(6, 7, 8), "nh": 3}]),
+        lambda: learned.fuse_pruned_device(img, atlas, atlas.clone(), small_out, 3, 2.5, 1.0,
+                                           learned.mlp_init(0, 0.0, 1.0)),
+        lambda: learned.mlp_train(img, torch.zeros(2, 6, 7), 3, 2.5, 1.0,
+                                  learned.mlp_init(0, 0.0, 1.0), steps=1),
+    ]
+    for call in calls:
+        with pytest.raises(ValueError):
+            call()
+    # matching shapes pass the check; host tensors are then refused as such
+    pct.check_volume_set(img, atlas, atlas.clone(), torch.zeros(6, 7, 8))
+    with pytest.raises(TypeError):
+        pct.fuse_device(img, atlas, atlas.clone(), torch.zeros(6, 7, 8), 3,
+                        stream=type("S", (), {"device": "cpu", "cuda_stream": 0})())

@@ -114,6 +114,36 @@ def test_matrixio_errors(tmp_path):
         pct.matrixio_read(p)
 
 
+def test_matrixio_header_products_that_wrap_are_refused(tmp_path):
+    # 65536^4 = 2^64 wraps a 64-bit product to 0: the reader must refuse the
+    # header before it allocates, not parse zero values into a zero-byte
+    # buffer and report 65536^3 dims
+    p = str(tmp_path / "wrap.txt")
+    with open(p, "w") as f:
+        f.write("65536\n65536\n65536\n65536\n")
+    with pytest.raises(pct.PctError) as e:
+        pct.matrixio_read(p)
+    assert e.value.status == _lib.PCT_ERR_TOO_LARGE
+    # one volume of 2^31 elements is refused too, however few atlases
+    with open(p, "w") as f:
+        f.write("1\n2048\n1024\n1024\n")
+    with pytest.raises(pct.PctError) as e:
+        pct.matrixio_read(p)
+    assert e.value.status == _lib.PCT_ERR_TOO_LARGE
+    # a sidecar with a wrapping header is ignored; the text is read instead
+    a = np.arange(8, dtype=np.float32)
+    pct.matrixio_write(p, a, 1, 2, 2, 2)
+    pct.matrixio_write_sidecar(p)
+    side = p + ".pctbin"
+    raw = bytearray(open(side, "rb").read())
+    hdr = np.frombuffer(bytes(raw[8:40]), np.int64).copy()
+    hdr[:] = 65536
+    raw[8:40] = hdr.tobytes()
+    open(side, "wb").write(bytes(raw))
+    got, dims = pct.matrixio_read(p)
+    assert dims == [2, 2, 2, 1] and np.array_equal(got, a)
+
+
 # ---- configuration file (cpp:908-1018) --------------------------------------
 
 def _write_set(tmp_path, img, atlas, label):

@@ -5,6 +5,8 @@ The reference's MATLAB build reads `atlas` / `atlas_label` structs and the
 target image from a MAT-file (src/CPU/NewPatchedBrainPureC.cpp:584-801) and
 converts MATLAB's column-major arrays so that element (x, y, z) of an
 nx x ny x nz array lands at (z*nx + x)*ny + y (cpp:377-474)."""
+import struct
+
 import numpy as np
 import pytest
 import scipy.io
@@ -117,3 +119,71 @@ def test_format_and_shape_errors(tmp_path):
     with pytest.raises(pct.PctError) as e:
         pct.mat_read(str(tmp_path / "absent.mat"))
     assert e.value.status == _lib.PCT_ERR_IO
+
+
+def _element(mi_type, payload):
+    pad = (-len(payload)) % 8
+    return struct.pack("<II", mi_type, len(payload)) + payload + b"\0" * pad
+
+
+def _matrix(name, cls, dims, payload=b"", fields=None):
+    """One miMATRIX element written by hand, so that its dims can be any
+    int32 values whatever its payload holds."""
+    body = _element(6, struct.pack("<II", cls, 0))                  # array flags
+    body += _element(5, struct.pack("<%di" % len(dims), *dims))      # dimensions
+    body += _element(1, name.encode())                               # name
+    if fields is not None:
+        flen = 32
+        body += _element(5, struct.pack("<i", flen))
+        body += _element(1, b"".join(f.encode().ljust(flen, b"\0") for f, _ in fields))
+        for _, sub in fields:
+            body += sub
+    else:
+        body += _element(7, payload)                                 # miSINGLE values
+    return _element(14, body)
+
+
+def _mat_file(path, elements):
+    head = b"MATLAB 5.0 MAT-file, crafted".ljust(116, b" ") + b"\0" * 8
+    head += struct.pack("<H", 0x0100) + b"IM"
+    with open(path, "wb") as f:
+        f.write(head + b"".join(elements))
+
+
+def test_dims_whose_product_wraps_are_refused(tmp_path):
+    # 2^22 * 2^21 * 2^21 = 2^64: a size_t product wraps to 0, so an unchecked
+    # reader would accept an empty payload and then fill 2^64 elements
+    dims = [1 << 22, 1 << 21, 1 << 21]
+    vol = _matrix("", 7, dims)
+    path = str(tmp_path / "wrap.mat")
+    _mat_file(path, [_matrix("img", 7, dims),
+                     _matrix("atlas", 2, [1, 1], fields=[("MRI1", vol)]),
+                     _matrix("atlas_label", 2, [1, 1], fields=[("CT1", vol)])])
+    with pytest.raises(pct.PctError) as e:
+        pct.mat_read(path)
+    assert e.value.status in (_lib.PCT_ERR_FORMAT, _lib.PCT_ERR_TOO_LARGE)
+    # a product that fits 64 bits but exceeds one volume's limit
+    dims = [2048, 1024, 1024]
+    vol = _matrix("", 7, dims)
+    _mat_file(path, [_matrix("img", 7, dims),
+                     _matrix("atlas", 2, [1, 1], fields=[("MRI1", vol)]),
+                     _matrix("atlas_label", 2, [1, 1], fields=[("CT1", vol)])])
+    with pytest.raises(pct.PctError) as e:
+        pct.mat_read(path)
+    assert e.value.status in (_lib.PCT_ERR_FORMAT, _lib.PCT_ERR_TOO_LARGE)
+
+
+def test_crafted_writer_matches_scipy_reader(tmp_path):
+    # the hand-written elements above are valid MAT-v5: a small crafted file
+    # reads back through the library with the values it holds
+    img = np.arange(24, dtype=np.float32).reshape(2, 3, 4)
+    col = img.transpose(2, 1, 0).astype("<f4").tobytes()   # MATLAB order of a 2x3x4 array
+    dims = [2, 3, 4]
+    path = str(tmp_path / "ok.mat")
+    _mat_file(path, [_matrix("img", 7, dims, col),
+                     _matrix("atlas", 2, [1, 1], fields=[("MRI1", _matrix("", 7, dims, col))]),
+                     _matrix("atlas_label", 2, [1, 1],
+                             fields=[("CT1", _matrix("", 7, dims, col))])])
+    assert np.array_equal(scipy.io.loadmat(path)["img"], img)
+    d = pct.mat_read(path)
+    assert np.array_equal(d["img"], _layout(img))
