@@ -8,14 +8,22 @@ inputs resident in HBM (tiled SM kernel + finalize pass).  `value` is
 full-volume voxels (222*222*112) per second.
 
 Extra fields:
-  roofline      the tiled fusion kernel against MI355X's FP32 vector peak;
-                algorithmic FLOPs = 86 per candidate-voxel pair (the
-                reference's direct formulation: 27 sub + 27 FMA + scale + exp
-                + 3 for the accumulation), pairs = interior voxels * n * NH^3,
-                kernel time from HIP events around each launch.
+  roofline      the tiled fusion kernel against MI355X's FP32 vector peak:
+                achieved = FP32 FLOP the kernel executes per launch (rocprof
+                SQ_INSTS_VALU_* counters x 64 lanes, FMA counted twice, from
+                profiles/, used only when the profile's source stamp matches
+                the kernel sources built here) / kernel time; frac =
+                achieved / peak; issue_frac = VALU instructions issued /
+                the non-packed issue ceiling.  effective_tflops counts the
+                reference's direct formulation instead (86 FLOP per
+                candidate-voxel pair: 27 sub + 27 FMA + scale + exp + 3 for
+                the accumulation), which the kernel's box-sum reuse beats.
+                Kernel time from HIP events around each launch.
   cpu_baseline  the reference CPU path (oracle/_ref, compiled from the
-                reference source) or the oracle port, OpenMP over z, timed on
-                a bounded crop of the same inputs and scaled to voxels/s.
+                reference source) or the oracle port, OpenMP over z with a
+                static schedule, timed on a crop of the same inputs that is
+                one round of planes for the thread count, and scaled by the
+                rounds and rows of the full volume.
   parity_sample float64 oracle vs the GPU output on sampled voxels.
   end_to_end    host buffers in and out around the fusion and the host post
                 chain, the region the reference's GPU program times.
@@ -24,8 +32,11 @@ Extra fields:
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
+import math
 import os
+import subprocess
 import sys
 import time
 
@@ -37,6 +48,39 @@ PEAK_HBM_GBS = 8000.0
 FLOPS_PER_CVP = 86
 METRIC = ("pseudo-CT voxels/s @222\u00d7222\u00d7112 n=10 NH=11: plain vs learned-pruned "
           "@\u22645HU MAE")
+
+
+# the files whose contents define the tiled fusion kernels: a profile is
+# attached to a run only when it was taken of a build of the same contents
+KERNEL_SOURCES = ("pseudoctimaging_amd/csrc/pct_tiled.h", "pseudoctimaging_amd/csrc/pct_fuse.hip",
+                  "pseudoctimaging_amd/csrc/Makefile")
+PROFILE = os.path.join("profiles", "fuse_tiled_222x222x112_n10_nh11_traffic.json")
+VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4   # wave64 FP32 instructions, non-packed
+
+
+def kernel_source_stamp() -> str:
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(rel.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
+def host_cpu_info() -> dict:
+    """CPUs this process may run on and the host's topology (lscpu)."""
+    info = {"affinity_cpus": len(os.sched_getaffinity(0)), "logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        info["model"] = kv.get("Model name")
+        sockets = int(kv.get("Socket(s)", "0") or 0)
+        cores = int(kv.get("Core(s) per socket", "0") or 0)
+        info["physical_cores"] = sockets * cores if sockets and cores else None
+        info["threads_per_core"] = int(kv.get("Thread(s) per core", "0") or 0) or None
+    except (OSError, ValueError, subprocess.SubprocessError):
+        info["model"] = None
+    return info
 
 
 def log(msg: str) -> None:
@@ -270,7 +314,6 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-planes", type=int, default=16)
     ap.add_argument("--cpu-rows", type=int, default=48)
     ap.add_argument("--parity-voxels", type=int, default=2000)
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
@@ -289,6 +332,12 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
+    # the CPUs this process may use, read before an OpenMP runtime binds the
+    # main thread to one of them
+    hostinfo = host_cpu_info()
+    # the CPU baseline's OpenMP threads spread over those CPUs (read when the
+    # OpenMP runtime starts, so before anything loads it)
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
     import numpy as np
     import torch
 
@@ -359,42 +408,52 @@ def main() -> int:
     log(f"{ms_per_step:.3f} ms/step, kernel {kernel_ms:.3f} ms, {value:.4e} voxels/s")
 
     flops = FLOPS_PER_CVP * cvp
-    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    effective = flops / (kernel_ms * 1e-3) / 1e12
+    kname = "fuse_tiled_xt" if cfg.get("form", 0) == 1 else "fuse_tiled"
+    timed_kernel = (f"{kname}<{cfg['seg_width']},{cfg['bx']},{cfg['bz']},{cfg['waves']},"
+                    f"{(nh - 1) // 2}>")
+    stamp = kernel_source_stamp()
     roofline = {
         "bound": "valu_fp32",
-        "achieved": round(achieved, 3),
+        "achieved": None,
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
-        "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+        "frac": None,
         "traffic": None,
-        "kernel": None,
+        "issue_frac": None,
+        "effective_tflops": round(effective, 3),
+        "effective_frac": round(effective / PEAK_FP32_TFLOPS, 4),
+        "kernel": timed_kernel,
         "kernel_ms": round(kernel_ms, 4),
-        "flops_per_launch": flops,
+        "algorithmic_flops_per_launch": flops,
         "cvp_per_launch": cvp,
         "algorithmic_bytes": 4 * full * (2 + 2 * n),
+        "source_stamp": stamp,
     }
-    tfile = os.path.join(ROOT, "profiles", "fuse_tiled_222x222x112_n10_nh11_traffic.json")
-    timed_kernel = (f"fuse_tiled<{cfg['seg_width']},{cfg['bx']},{cfg['bz']},{cfg['waves']},"
-                    f"{(nh - 1) // 2}>")
-    roofline["kernel"] = timed_kernel
+    tfile = os.path.join(ROOT, PROFILE)
     prof = None
     if (nz, nx, ny, n, nh) == (112, 222, 222, 10, 11) and os.path.exists(tfile):
         with open(tfile) as f:
             prof = json.load(f)
+        why = None
         if prof.get("kernel") != timed_kernel:
-            # counters of another tile shape do not describe this launch
-            roofline["traffic_note"] = (f"profiled kernel {prof.get('kernel')} is not the timed "
-                                        f"kernel {timed_kernel}")
+            why = f"profiled kernel {prof.get('kernel')} is not the timed kernel {timed_kernel}"
+        elif prof.get("source_stamp") != stamp:
+            why = (f"profile taken of kernel sources {prof.get('source_stamp')}, built here "
+                   f"{stamp}")
+        if why:
+            # counters of another kernel do not describe this launch
+            roofline["counters_note"] = why
             prof = None
     if prof is not None:
         roofline["traffic"] = prof.get("hbm_bytes_per_launch")
-        if "valu" in prof:
-            v = prof["valu"]
-            roofline["counters"] = {
-                "valu_issue_fraction": v["valu_issue_fraction"],
-                "executed_fp32_tflops": round(v["executed_fp32_flop_per_launch"]
-                                              / (kernel_ms * 1e-3) / 1e12, 2),
-                "source": os.path.relpath(tfile, ROOT)}
+        v = prof["valu"]
+        executed = v["executed_fp32_flop_per_launch"] / (kernel_ms * 1e-3) / 1e12
+        roofline["achieved"] = round(executed, 3)
+        roofline["frac"] = round(executed / PEAK_FP32_TFLOPS, 4)
+        roofline["issue_frac"] = round((v["SQ_INSTS_VALU"] - v["SQ_INSTS_VALU_TRANS_F32"])
+                                       / (kernel_ms * 1e-3) / VALU_ISSUE_PER_S, 4)
+        roofline["counters_source"] = PROFILE
 
     out = d_out.cpu().numpy()
     parity = None
@@ -442,19 +501,30 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        planes = min(args.cpu_planes, interior(nz, r))
-        rows = min(args.cpu_rows, interior(nx, r))
-        log(f"cpu baseline: {planes} planes x {rows} rows crop, {threads} threads")
+        threads = min(16, hostinfo["affinity_cpus"])
+        iz, ix = interior(nz, r), interior(nx, r)
+        # the reference parallelises z with OpenMP's static schedule (cpp:1374):
+        # a crop of one plane per thread is one round, and the full volume
+        # takes ceil(iz / threads) rounds
+        planes = min(threads, iz)
+        rows = min(args.cpu_rows, ix)
+        log(f"cpu baseline: {planes} planes x {rows} rows crop, {threads} threads, "
+            f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}")
         kind, dt, vox_s = cpu_baseline(img, amri, lab, nh, sigma, beta, planes, rows, threads)
-        full_time = dt * n_int / vox_s
+        rounds_full = math.ceil(iz / threads)
+        rounds_crop = math.ceil(planes / threads)
+        full_time = dt * (rounds_full / rounds_crop) * (ix / rows)
         cpu = {"value": full / full_time, "unit": "voxels/s", "cores": threads, "kind": kind,
                "sample": f"crop of the same inputs: {planes} interior z planes x {rows} "
                          f"interior x rows x all y ({vox_s} interior voxels, n={n}, NH={nh}) "
-                         f"in {dt:.2f}s, scaled by interior voxels to the full volume",
-               "host_logical_cpus": os.cpu_count(),
-               "parallelism": f"OpenMP over z as the reference (cpp:1374): at most "
-                              f"{interior(nz, r)} threads have work at full size"}
+                         f"in {dt:.2f}s; the full volume is {rounds_full} rounds of "
+                         f"{threads} planes under the static schedule and {ix / rows:.2f}x "
+                         f"the rows",
+               "seconds_full_volume": full_time,
+               "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+               "host": hostinfo,
+               "parallelism": f"OpenMP over z as the reference (cpp:1374), static schedule: "
+                              f"at most {iz} threads have work at full size"}
         log(f"cpu baseline: {cpu}")
 
     line = {

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PCT_ABI_VERSION 1
+#define PCT_ABI_VERSION 2
 
 /* Same object as hipStream_t; declared here so that the header needs no HIP
  * headers.  NULL means the default stream. */
@@ -144,6 +144,10 @@ typedef struct pct_tile_config {
     long long items;   /* work items (workgroups) launched */
     double est_cost;   /* cost-model estimate (arbitrary units) */
     double measured_ms;/* <0 if not measured */
+    int form;          /* 0: squared differences per candidate; 1: cross terms
+                          (2 T'A' products, per-voxel 2^-AA folded into the
+                          staged CT; squared differences where the value range
+                          needs them) */
 } pct_tile_config;
 
 /* Number of tiled-kernel configurations compiled into the library. */

@@ -42,7 +42,8 @@ class TileConfig(Structure):
     """pct_tile_config"""
     _fields_ = [("id", c_int), ("seg_width", c_int), ("bx", c_int), ("bz", c_int),
                 ("waves", c_int), ("atlas_group", c_int), ("lds_bytes", c_int),
-                ("items", c_longlong), ("est_cost", c_double), ("measured_ms", c_double)]
+                ("items", c_longlong), ("est_cost", c_double), ("measured_ms", c_double),
+                ("form", c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -180,7 +180,7 @@ static bool plan_batch(const pct_subject* s, int n, const pct_batch_choice& ch, 
         if (!make_geometry(&s[i].pb, sid, std::min(grp, s[i].pb.n_atlas), &g)) return false;
         p->geo[i] = g;
         p->ws_off[i] = off;
-        off += tiled_ws_bytes(&s[i].pb, g.n_groups);
+        off += tiled_ws_bytes(&s[i].pb, g.n_groups, sid);
         item_cost[i] = g.group;
         if (g.n_items > 0) by_hw[hw].push_back(i);
     }
@@ -641,6 +641,7 @@ extern "C" int pct_autotune_batch(const pct_subject* subjects, int n_subjects, i
             best->bx = sh.bx;
             best->bz = sh.bz;
             best->waves = sh.waves;
+            best->form = sh.form;
             best->atlas_group = ch.atlas_group[top];
         }
         best->est_cost = -1;

@@ -112,6 +112,53 @@ __global__ __launch_bounds__(256) void fuse_direct(const float* __restrict__ img
     out[idx] = fused_value(P, S);
 }
 
+// AA of every voxel of atlases [0, n_atlas): the 3x3x3 sum of (scale * A)^2,
+// zero outside the volume.  A workgroup owns 64 y x 4 x columns and sweeps z:
+// each plane's (4+2) x (64+2) tile goes through LDS for the in-plane 3x3
+// sums, which enter a running sum over three planes.
+constexpr int kSqBoxY = 64, kSqBoxX = 4;
+__global__ __launch_bounds__(kSqBoxY * kSqBoxX) void atlas_sq_box(const float* __restrict__ atlas,
+                                                                  long long N, int nx, int ny,
+                                                                  int nz, int n_atlas,
+                                                                  float scale,
+                                                                  float* __restrict__ aa) {
+    __shared__ float tile[kSqBoxX + 2][kSqBoxY + 2];
+    const int ly = threadIdx.x % kSqBoxY, lx = threadIdx.x / kSqBoxY;
+    const int y0 = blockIdx.x * kSqBoxY, x0 = blockIdx.y * kSqBoxX;
+    const long long plane = (long long)nx * ny;
+    const int y = y0 + ly, x = x0 + lx;
+    for (int at = blockIdx.z; at < n_atlas; at += gridDim.z) {
+    const float* __restrict__ A = atlas + (long long)at * N;
+    float* __restrict__ out = aa + (long long)at * N;
+    float s_prev = 0.f, s_cur = 0.f;   // in-plane sums of planes z-2 and z-1
+    for (int z = 0; z <= nz; ++z) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < (kSqBoxX + 2) * (kSqBoxY + 2); e += kSqBoxY * kSqBoxX) {
+            const int rx = e / (kSqBoxY + 2), cy = e - rx * (kSqBoxY + 2);
+            const int xx = x0 - 1 + rx, yy = y0 - 1 + cy;
+            float v = 0.f;
+            if (z < nz && xx >= 0 && xx < nx && yy >= 0 && yy < ny)
+                v = A[z * plane + (long long)xx * ny + yy] * scale;
+            tile[rx][cy] = v;
+        }
+        __syncthreads();
+        float s = 0.f;
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+                const float v = tile[lx + dx][ly + dy];
+                s = fmaf(v, v, s);
+            }
+        // plane z-1 is complete: its sum over planes z-2, z-1, z
+        if (z >= 1 && x < nx && y < ny)
+            out[(z - 1) * plane + (long long)x * ny + y] = (s_prev + s_cur) + s;
+        s_prev = s_cur;
+        s_cur = s;
+    }
+    }
+}
+
 // cpp:1583-1593, float32, same operation order.
 __global__ void normalize_labels_kernel(float* __restrict__ v, long long n) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
@@ -135,21 +182,24 @@ struct TiledEntry {
     mutable int occupancy;
 };
 
-template <int W, int BX, int BZ, int NW, int HW>
+template <int W, int BX, int BZ, int NW, int HW, int FORM>
 TiledEntry make_entry() {
     using C = TiledCfg<W, BX, BZ, NW, HW>;
-    return TiledEntry{{W, BX, BZ, NW}, HW, &fuse_tiled<W, BX, BZ, NW, HW>, C::LDS_BYTES, 0};
+    if (FORM == 1)
+        return TiledEntry{{W, BX, BZ, NW, 1}, HW, &fuse_tiled_xt<W, BX, BZ, NW, HW>,
+                          XtCfg<C>::LDS_BYTES, 0};
+    return TiledEntry{{W, BX, BZ, NW, 0}, HW, &fuse_tiled<W, BX, BZ, NW, HW>, C::LDS_BYTES, 0};
 }
 
-template <int W, int BX, int BZ, int NW>
+template <int W, int BX, int BZ, int NW, int FORM = 0>
 void add_shape(std::vector<TiledEntry>& v) {
-    v.push_back(make_entry<W, BX, BZ, NW, 1>());
-    v.push_back(make_entry<W, BX, BZ, NW, 2>());
-    v.push_back(make_entry<W, BX, BZ, NW, 3>());
-    v.push_back(make_entry<W, BX, BZ, NW, 4>());
-    v.push_back(make_entry<W, BX, BZ, NW, 5>());
-    v.push_back(make_entry<W, BX, BZ, NW, 6>());
-    v.push_back(make_entry<W, BX, BZ, NW, 7>());
+    v.push_back(make_entry<W, BX, BZ, NW, 1, FORM>());
+    v.push_back(make_entry<W, BX, BZ, NW, 2, FORM>());
+    v.push_back(make_entry<W, BX, BZ, NW, 3, FORM>());
+    v.push_back(make_entry<W, BX, BZ, NW, 4, FORM>());
+    v.push_back(make_entry<W, BX, BZ, NW, 5, FORM>());
+    v.push_back(make_entry<W, BX, BZ, NW, 6, FORM>());
+    v.push_back(make_entry<W, BX, BZ, NW, 7, FORM>());
 }
 
 // The shapes compiled into the library (index = pct_tile_config.id).
@@ -170,6 +220,12 @@ const std::vector<TileShape>& shapes() {
         {32, 2, 2, 2},   // 12: 8 x rows, 2 planes, two waves; eight workgroups per CU
         {32, 2, 4, 8},   // 13: 32 x rows, 4 planes, eight waves
         {32, 4, 2, 8},   // 14: 64 x rows, 2 planes, eight waves
+        // cross-term form (slab: BZ + 2 MRI planes and BZ planes of (L~, E) pairs)
+        {32, 2, 4, 4, 1},   // 15: 16 x rows, 4 planes; two workgroups per CU up to NH=11
+        {32, 2, 2, 4, 1},   // 16: 16 x rows, 2 planes; four workgroups per CU up to NH=11
+        {32, 2, 2, 8, 1},   // 17: 32 x rows, 2 planes, eight waves
+        {32, 4, 2, 4, 1},   // 18: 32 x rows, 2 planes
+        {64, 2, 2, 4, 1},   // 19: 8 x rows of 62-wide segments, 2 planes
     };
     return s;
 }
@@ -192,6 +248,11 @@ static const std::vector<TiledEntry>& entries() {
         add_shape<32, 2, 2, 2>(v);
         add_shape<32, 2, 4, 8>(v);
         add_shape<32, 4, 2, 8>(v);
+        add_shape<32, 2, 4, 4, 1>(v);
+        add_shape<32, 2, 2, 4, 1>(v);
+        add_shape<32, 2, 2, 8, 1>(v);
+        add_shape<32, 4, 2, 4, 1>(v);
+        add_shape<64, 2, 2, 4, 1>(v);
         return v;
     }();
     return e;
@@ -270,11 +331,20 @@ double model_item_time(const pct_problem* pb, int shape_id, int group, TileGeome
     const TileShape& s = shapes()[shape_id];
     const double bx = s.bx, bz = s.bz;
     const double nh = 2 * g.hw + 1;
-    const double step = 1.7 * (bz + 2) * (bx + 2) + 1.5 * (bz + 2) * bx + 6.5 * bz * bx +
-                        3.0 * bz * bx;
-    // staging per dz step: address, load, scale, LDS store per element
+    // cross-term shapes: per step 2 (BZ+2) BX FMA-chain instructions for the
+    // x sums, 1.5 BZ BX for z, 2 BZ BX DPP adds for y, 2 BZ BX FMAs for P and
+    // S, BZ BX exp and one address add; scaled like the fit below (measured:
+    // {32, 2, 2, 4} at 222x222x112, n=10, NH=11 runs at 85 % of its VALU
+    // issue rate, its squared-difference twin at 100 %)
+    const double step = s.form == 1
+                            ? 1.28 / 0.85 * (2.0 * (bz + 2) * bx + 6.5 * bz * bx + 1.0)
+                            : 1.7 * (bz + 2) * (bx + 2) + 1.5 * (bz + 2) * bx + 6.5 * bz * bx +
+                                  3.0 * bz * bx;
+    // staging per dz step: address, load, scale, LDS store per element (the
+    // cross-term slab also loads AA and forms (L~, E) for its CT planes)
     const int S = 64 / s.seg_width;
-    const double rows = (2 * bz + 2) * (s.waves * S * s.bx + 2 + 2 * g.hw);
+    const double rows = (s.form == 1 ? 3 * bz + 2 : 2 * bz + 2) *
+                        (s.waves * S * s.bx + 2 + 2 * g.hw);
     const double stage = rows / s.waves * 8.0 * ((s.seg_width + 2 * g.hw + 63) / 64);
     // one-segment (64-lane) shapes measured about 1.5x slower than their
     // instruction counts predict
@@ -301,9 +371,14 @@ static double model_cost(const pct_problem* pb, int shape_id, int group, TileGeo
 // while the best shapes of a window lie within 5 % of each other.
 int preferred_shape(int hw) {
     switch (hw) {
-        case 3: return 11;   // {32, 2, 2, 4}: 0.42 ms vs 0.45 next (160x160x96, n=1)
-        case 4: return 9;    // {64, 2, 4, 4}: 21.2 ms vs 21.7 next (256x256x128, n=10)
-        case 5: return 7;    // {32, 2, 4, 4}: 22.3 ms vs 23.3 next (222x222x112, n=10)
+        // the cross-term {32, 2, 2, 4} (16) against the fastest squared-difference
+        // shape of each window: 0.37 vs 0.44 ms (160x160x96, n=1), 1.07 vs 1.17
+        // (128x128x64, n=10); 2.67 vs 2.72 (192x192x96, n=3), 19.1 vs 21.2
+        // (256x256x128, n=10); 10.0 vs 11.6 (222x222x112, n=5), 19.6 vs 22.3
+        // (n=10)
+        case 3: return 16;
+        case 4: return 16;
+        case 5: return 16;
         default: return -1;
     }
 }
@@ -427,9 +502,32 @@ int ensure_workspace(int dev, size_t bytes, void** ptr) {
     return PCT_OK;
 }
 
-size_t tiled_ws_bytes(const pct_problem* pb, int n_groups) {
+bool is_xterm_shape(int shape_id) {
+    return shape_id >= 0 && shape_id < (int)shapes().size() && shapes()[shape_id].form == 1;
+}
+
+size_t tiled_aa_offset(const pct_problem* pb, int n_groups) {
     const size_t N = (size_t)pb->nx * pb->ny * pb->nz;
     return ((2 * (size_t)n_groups * N * sizeof(float)) + 255) & ~(size_t)255;
+}
+
+size_t tiled_ws_bytes(const pct_problem* pb, int n_groups, int shape_id) {
+    const size_t N = (size_t)pb->nx * pb->ny * pb->nz;
+    size_t bytes = tiled_aa_offset(pb, n_groups);
+    if (is_xterm_shape(shape_id))
+        bytes += ((size_t)pb->n_atlas * N * sizeof(float) + 255) & ~(size_t)255;
+    return bytes;
+}
+
+int launch_atlas_aa(const pct_problem* pb, const float* atlas, int n_atlas, float scale,
+                    float* aa, hipStream_t stream) {
+    if (n_atlas <= 0) return PCT_OK;
+    const long long N = (long long)pb->nx * pb->ny * pb->nz;
+    const dim3 grid((pb->ny + kSqBoxY - 1) / kSqBoxY, (pb->nx + kSqBoxX - 1) / kSqBoxX,
+                    (unsigned)std::min(n_atlas, 65535));
+    hipLaunchKernelGGL(atlas_sq_box, grid, dim3(kSqBoxY * kSqBoxX), 0, stream, atlas, N, pb->nx,
+                       pb->ny, pb->nz, n_atlas, scale, aa);
+    return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;
 }
 
 static bool valid_choice(int choice) {
@@ -474,6 +572,7 @@ TiledArgs make_tiled_args(const pct_problem* pb, const TileGeometry& g, const fl
     const double div = pct_weight_divisor(pb->sigma, pb->beta);
     a.scale = (float)std::sqrt(1.4426950408889634 / div);
     a.only = nullptr;
+    a.aa = nullptr;
     return a;
 }
 
@@ -482,6 +581,7 @@ TiledArgs make_tiled_args(const pct_problem* pb, const TileGeometry& g, const fl
 int launch_tiled_items(const TiledArgs& a, int shape_id, int hw, hipStream_t stream) {
     const TiledEntry* e = find_entry(shape_id, hw);
     if (!e) return PCT_ERR_UNSUPPORTED;
+    if (is_xterm_shape(shape_id) && !a.aa) return PCT_ERR_INVALID;   // AA pass not run
     if (a.n_items <= 0) return PCT_OK;
     TiledArgs copy = a;
     void* kargs[] = {&copy};
@@ -532,8 +632,15 @@ static int launch_tiled(const pct_problem* pb, const float* img, const float* at
                         int shape_id, const TileGeometry& g, hipStream_t stream) {
     const TiledEntry* e = find_entry(shape_id, g.hw);
     if (!e) return PCT_ERR_UNSUPPORTED;
-    if (ws_bytes < tiled_ws_bytes(pb, g.n_groups)) return PCT_ERR_INVALID;
+    if (ws_bytes < tiled_ws_bytes(pb, g.n_groups, shape_id)) return PCT_ERR_INVALID;
     TiledArgs a = make_tiled_args(pb, g, img, atlas, label, static_cast<float*>(ws));
+    if (is_xterm_shape(shape_id) && g.n_items > 0) {
+        float* aa = reinterpret_cast<float*>(static_cast<char*>(ws) +
+                                             tiled_aa_offset(pb, g.n_groups));
+        const int rc = launch_atlas_aa(pb, atlas, pb->n_atlas, a.scale, aa, stream);
+        if (rc) return rc;
+        a.aa = aa;
+    }
     if (g.n_items > 0) {
         void* kargs[] = {&a};
         TimedLaunch timed(stream);
@@ -626,7 +733,7 @@ extern "C" size_t pct_fuse_workspace_size(const pct_problem* pb, int choice) {
     int sid = 0, grp = 1;
     TileGeometry g;
     if (!choose(pb, &sid, &grp, nullptr, &g)) return 0;
-    return tiled_ws_bytes(pb, g.n_groups);
+    return tiled_ws_bytes(pb, g.n_groups, sid);
 }
 
 static int fuse_dev_impl(const pct_problem* pb, const float* img, const float* atlas,
@@ -643,7 +750,7 @@ static int fuse_dev_impl(const pct_problem* pb, const float* img, const float* a
     int sid = 0, grp = 1;
     TileGeometry g;
     if (!choose(pb, &sid, &grp, nullptr, &g)) return PCT_ERR_UNSUPPORTED;
-    const size_t need = tiled_ws_bytes(pb, g.n_groups);
+    const size_t need = tiled_ws_bytes(pb, g.n_groups, sid);
     if (own_ws) {
         rc = ensure_workspace(dev, need, &ws);
         if (rc) return rc;
@@ -681,6 +788,11 @@ struct HostCallState {
     std::mutex mu;
     float *img = nullptr, *atlas = nullptr, *label = nullptr, *out = nullptr;
     size_t n_cap = 0, na_cap = 0;
+    // workspace of the host-buffer calls (partial sums, AA, post-chain
+    // lists): their own, so that they never share memory with device-buffer
+    // calls still running on the caller's streams
+    void* ws = nullptr;
+    size_t ws_cap = 0;
     // chunk kernels alternate between two compute streams, so that the tail
     // of one chunk overlaps the start of the next
     hipStream_t compute = nullptr, compute2 = nullptr, copy = nullptr;
@@ -778,8 +890,27 @@ static void free_host_buffers(HostCallState& st) {
     if (st.atlas) (void)hipFree(st.atlas);
     if (st.label) (void)hipFree(st.label);
     if (st.out) (void)hipFree(st.out);
+    if (st.ws) (void)hipFree(st.ws);
     st.img = st.atlas = st.label = st.out = nullptr;
-    st.n_cap = st.na_cap = 0;
+    st.ws = nullptr;
+    st.n_cap = st.na_cap = st.ws_cap = 0;
+}
+
+// Grows the host-call workspace to `bytes` (the previous call drained every
+// stream, so nothing still uses it).
+static int host_workspace(HostCallState& st, size_t bytes, void** ws) {
+    if (st.ws_cap < bytes) {
+        if (st.ws) (void)hipFree(st.ws);
+        st.ws = nullptr;
+        st.ws_cap = 0;
+        if (hipMalloc(&st.ws, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return PCT_ERR_NO_MEMORY;
+        }
+        st.ws_cap = bytes;
+    }
+    *ws = st.ws;
+    return PCT_OK;
 }
 
 static int prepare_host_state(HostCallState& st, size_t N, size_t NA) {
@@ -806,7 +937,7 @@ static int prepare_host_state(HostCallState& st, size_t N, size_t NA) {
     for (int b = 0; b < kPinBufs; ++b) st.pin_busy[b] = false;
     st.pin_next = 0;
     if (st.n_cap < N || st.na_cap < NA) {
-        free_host_buffers(st);
+        free_host_buffers(st);   // the workspace too; host_workspace regrows it
         if (hipMalloc(&st.img, N * 4) != hipSuccess || hipMalloc(&st.atlas, NA * 4) != hipSuccess ||
             hipMalloc(&st.label, NA * 4) != hipSuccess || hipMalloc(&st.out, N * 4) != hipSuccess) {
             (void)hipGetLastError();
@@ -848,7 +979,16 @@ static int fuse_host_impl(const pct_problem* pb, const float* img, const float* 
     TileGeometry g;
     const bool tiled = is_tiled_choice(pb, choice) && choose(pb, &sid, &grp, nullptr, &g);
     void* ws = nullptr;
-    if (tiled) rc = ensure_workspace(dev, tiled_ws_bytes(pb, g.n_groups), &ws);
+    {
+        size_t need = post ? postprocess_ws_bytes(pb->nh, pb->nx, pb->ny, pb->nz) : 0;
+        if (tiled) need = std::max(need, tiled_ws_bytes(pb, g.n_groups, sid));
+        if (!need) need = 256;
+        rc = host_workspace(st, need, &ws);
+    }
+    float* aa = tiled && is_xterm_shape(sid)
+                    ? reinterpret_cast<float*>(static_cast<char*>(ws) +
+                                               tiled_aa_offset(pb, g.n_groups))
+                    : nullptr;
     // cu:1687-1699: host->device copies
     if (!rc) rc = stage_h2d(st, st.img, img, N * 4, cs);
     if (!rc && tiled) {
@@ -871,7 +1011,13 @@ static int fuse_host_impl(const pct_problem* pb, const float* img, const float* 
                                           static_cast<float*>(ws) + 2 * (size_t)g0 * N);
             a.n_atlas = a1 - a0;
             a.n_items = (int)(g.n_tiles * (g1 - g0));
-            rc = launch_tiled_items(a, sid, g.hw, (c & 1) ? st.compute2 : s);
+            hipStream_t cstream = (c & 1) ? st.compute2 : s;
+            if (aa) {
+                // AA of the chunk's atlases, computed as the single launch does
+                rc = launch_atlas_aa(pb, st.atlas + off, a1 - a0, a.scale, aa + off, cstream);
+                a.aa = aa + off;
+            }
+            if (!rc) rc = launch_tiled_items(a, sid, g.hw, cstream);
         }
         if (!rc && (hipEventRecord(st.join, st.compute2) != hipSuccess ||
                     hipStreamWaitEvent(s, st.join, 0) != hipSuccess))
@@ -883,12 +1029,10 @@ static int fuse_host_impl(const pct_problem* pb, const float* img, const float* 
         if (!rc && (hipEventRecord(st.ev[0], cs) != hipSuccess ||
                     hipStreamWaitEvent(s, st.ev[0], 0) != hipSuccess))
             rc = PCT_ERR_HIP;
-        if (!rc) rc = fuse_dev_impl(pb, st.img, st.atlas, st.label, st.out, choice, nullptr, 0,
-                                    true, s);
+        if (!rc) rc = fuse_dev_impl(pb, st.img, st.atlas, st.label, st.out, choice, ws,
+                                    st.ws_cap, false, s);
     }
-    if (!rc && post)
-        rc = pct_postprocess_dev(pb->nh, pb->nx, pb->ny, pb->nz, st.out,
-                                 reinterpret_cast<pct_stream_t>(s));
+    if (!rc && post) rc = postprocess_dev_ws(pb->nh, pb->nx, pb->ny, pb->nz, st.out, ws, s);
     if (!rc) rc = stage_d2h(st, out, st.out, N * 4, s);
     // every stream is drained before returning, also after an error
     if (hipStreamSynchronize(cs) != hipSuccess && !rc) rc = PCT_ERR_HIP;
@@ -954,6 +1098,7 @@ static void fill_config(const pct_problem* pb, int sid, int grp, double cost,
     out->items = g.n_items;
     out->est_cost = cost;
     out->measured_ms = -1;
+    out->form = s.form;
     (void)pb;
     (void)grp;
 }
@@ -1015,7 +1160,7 @@ extern "C" int pct_autotune(const pct_problem* pb, const float* d_img, const flo
             TileGeometry g;
             if (!make_geometry(pb, sid, grp, &g)) continue;
             void* ws = nullptr;
-            const size_t need = tiled_ws_bytes(pb, g.n_groups);
+            const size_t need = tiled_ws_bytes(pb, g.n_groups, sid);
             rc = ensure_workspace(dev, need, &ws);
             if (rc) break;
             // warm-up

@@ -18,6 +18,7 @@ struct TileShape {
     int seg_width;  // lanes per y segment
     int bx, bz;     // per-lane output block
     int waves;      // wavefronts per workgroup
+    int form = 0;   // 0: squared differences; 1: cross terms where the range allows
 };
 
 // Geometry of one tiled launch, derived from the problem and a TileShape.

@@ -208,20 +208,30 @@ static int post_checks(int nh, int nx, int ny, int nz, const float* d_arr) {
     return PCT_OK;
 }
 
-static int enqueue_fill(int nh, int nx, int ny, int nz, float* d_arr, hipStream_t s) {
-    int dev = 0;
-    int rc = check_device(&dev);
-    if (rc) return rc;
+// Interior voxel count of the post chain's NaN fill.
+static long long fill_interior(int nh, int nx, int ny, int nz) {
     const int lo = (nh - 1) / 2 + 1;
     const long long iz = nz - 2 * lo, ix = nx - 2 * lo, iy = ny - 2 * lo;
-    const long long interior = (iz > 0 && ix > 0 && iy > 0) ? iz * ix * iy : 0;
+    return (iz > 0 && ix > 0 && iy > 0) ? iz * ix * iy : 0;
+}
+
+// Workspace of the NaN fill: a counter, two NaN lists and per-chunk counts
+// and offsets.
+static size_t fill_ws_bytes(long long interior) {
+    const size_t list_bytes = ((size_t)interior * 4 + 255) & ~(size_t)255;
+    const long long nb = (interior + kListChunk - 1) / kListChunk;
+    const size_t block_bytes = ((size_t)nb * 4 + 255) & ~(size_t)255;
+    return 256 + 2 * list_bytes + 2 * block_bytes;
+}
+
+static int enqueue_fill(int nh, int nx, int ny, int nz, float* d_arr, void* ws, hipStream_t s) {
+    const int lo = (nh - 1) / 2 + 1;
+    const long long iz = nz - 2 * lo, ix = nx - 2 * lo, iy = ny - 2 * lo;
+    const long long interior = fill_interior(nh, nx, ny, nz);
     if (interior == 0) return PCT_OK;
     const size_t list_bytes = ((size_t)interior * 4 + 255) & ~(size_t)255;
     const long long nb = (interior + kListChunk - 1) / kListChunk;
     const size_t block_bytes = ((size_t)nb * 4 + 255) & ~(size_t)255;
-    void* ws = nullptr;
-    rc = ensure_workspace(dev, 256 + 2 * list_bytes + 2 * block_bytes, &ws);
-    if (rc) return rc;
     char* w = static_cast<char*>(ws);
     int* count = reinterpret_cast<int*>(w);
     int* la = reinterpret_cast<int*>(w + 256);
@@ -239,6 +249,38 @@ static int enqueue_fill(int nh, int nx, int ny, int nz, float* d_arr, hipStream_
     return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;
 }
 
+// The fill on the library's per-device workspace.
+static int enqueue_fill_lib(int nh, int nx, int ny, int nz, float* d_arr, hipStream_t s) {
+    int dev = 0;
+    int rc = check_device(&dev);
+    if (rc) return rc;
+    const long long interior = fill_interior(nh, nx, ny, nz);
+    if (interior == 0) return PCT_OK;
+    void* ws = nullptr;
+    rc = ensure_workspace(dev, fill_ws_bytes(interior), &ws);
+    if (rc) return rc;
+    return enqueue_fill(nh, nx, ny, nz, d_arr, ws, s);
+}
+
+static int enqueue_round_hu(int nx, int ny, int nz, float* d_arr, hipStream_t s) {
+    const long long n = (long long)nx * ny * nz;
+    const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(round_hu_kernel, dim3(blocks), dim3(256), 0, s, d_arr, n, 1);
+    return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;
+}
+
+size_t postprocess_ws_bytes(int nh, int nx, int ny, int nz) {
+    return fill_ws_bytes(fill_interior(nh, nx, ny, nz));
+}
+
+int postprocess_dev_ws(int nh, int nx, int ny, int nz, float* d_arr, void* ws, hipStream_t s) {
+    int rc = post_checks(nh, nx, ny, nz, d_arr);
+    if (rc) return rc;
+    rc = enqueue_fill(nh, nx, ny, nz, d_arr, ws, s);
+    if (rc) return rc;
+    return enqueue_round_hu(nx, ny, nz, d_arr, s);
+}
+
 }  // namespace pct
 
 using namespace pct;
@@ -247,7 +289,7 @@ extern "C" int pct_regularize_dev(int nh, int nx, int ny, int nz, float* d_arr,
                                   pct_stream_t stream) {
     const int rc = post_checks(nh, nx, ny, nz, d_arr);
     if (rc) return rc;
-    return enqueue_fill(nh, nx, ny, nz, d_arr, reinterpret_cast<hipStream_t>(stream));
+    return enqueue_fill_lib(nh, nx, ny, nz, d_arr, reinterpret_cast<hipStream_t>(stream));
 }
 
 extern "C" int pct_postprocess_dev(int nh, int nx, int ny, int nz, float* d_arr,
@@ -255,10 +297,7 @@ extern "C" int pct_postprocess_dev(int nh, int nx, int ny, int nz, float* d_arr,
     int rc = post_checks(nh, nx, ny, nz, d_arr);
     if (rc) return rc;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    rc = enqueue_fill(nh, nx, ny, nz, d_arr, s);
+    rc = enqueue_fill_lib(nh, nx, ny, nz, d_arr, s);
     if (rc) return rc;
-    const long long n = (long long)nx * ny * nz;
-    const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 65536);
-    hipLaunchKernelGGL(round_hu_kernel, dim3(blocks), dim3(256), 0, s, d_arr, n, 1);
-    return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;
+    return enqueue_round_hu(nx, ny, nz, d_arr, s);
 }

@@ -59,10 +59,14 @@ struct TiledArgs {
     float scale;      // sqrt(log2(e)/div)
     // optional item filter: when set, items whose byte is 0 are skipped
     const unsigned char* only;
+    // cross-term shapes only: AA of every atlas voxel ([n_atlas][N]), the
+    // 3x3x3 sum of the squared scaled atlas MRI (atlas_sq_box)
+    const float* aa;
 };
 
 template <int W, int BX, int BZ, int NW, int HW>
 struct TiledCfg {
+    static constexpr int BXv = BX, BZv = BZ, HWv = HW, NWv = NW;
     static constexpr int S = 64 / W;           // segments per wave
     static constexpr int NSEG = NW * S;        // segments per workgroup (stacked in x)
     static constexpr int WX = NSEG * BX;       // output rows per workgroup
@@ -415,6 +419,482 @@ __global__ __launch_bounds__(NW * 64) void fuse_tiled(TiledArgs args) {
 }
 
 // ---------------------------------------------------------------------------
+// tiled kernel, cross-term form
+// ---------------------------------------------------------------------------
+//
+// With T' and A' the scaled target and atlas MRI, the patch SSD of output v
+// and candidate d expands as
+//     SSD' = TT(v) + AA(v+d) - C(v, d),
+//     TT = sum_p T'(v+p)^2,  AA = sum_p A'(v+d+p)^2,  C = sum_p 2 T'(v+p) A'(v+d+p),
+// so the weight is w = 2^-SSD' = 2^-TT(v) * 2^C * 2^-AA(v+d).  2^-TT(v) is
+// one factor per output: it is applied to P and S once, after the last
+// candidate.  2^-AA(v+d) depends only on the atlas position: AA comes from
+// one pass over the atlases per call (atlas_sq_box), and each slab carries
+// the pair (L~, E) = (L * 2^-AA, 2^-AA) per position in place of the CT.
+// Per candidate the lane forms C with the same separable box sums as the
+// squared-difference kernel, its x sums now FMA chains of products (one
+// instruction per voxel instead of a difference and a square), then
+// w~ = 2^C, P += w~ L~, S += w~ E.
+//
+// Every LDS read is 64 bits wide, which costs one instruction slot like a
+// 32-bit read on gfx950: the MRI planes are interleaved in pairs, so a read
+// returns two planes at a position, and (L~, E) is one pair.  All reads of a
+// candidate address one base register (plane offsets ride in the 16-bit
+// immediates), one add per candidate.
+//
+// The expanded form loses absolute precision in proportion to TT + AA, and
+// 2^C must not overflow, so it is used only where every |T'| and |A'| is at
+// most kXtBound: then C <= 2 * 27 * kXtBound^2 = 3.4 and the exponent error
+// stays within a few float ulps of 3.4.  An item whose target values exceed
+// the bound runs the squared-difference form throughout; a slab whose atlas
+// values (or CT values, beyond kXtLabelBound) exceed it is weighed with the
+// squared differences and w = 2^(TT - SSD'), the same scaled weight.
+
+constexpr float kXtBound = 0.25f;
+constexpr float kXtLabelBound = 1.8446744e19f;   // 2^64: bounds P / 2^-TT
+
+// Native two-float vector: its loads and stores stay single 64-bit LDS
+// accesses (HIP's float2 struct is copied member by member).
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) const f32x2 lds_pair_t;
+
+// LDS byte address of an element of the dynamic shared array.
+__device__ __forceinline__ unsigned lds_byte_address(const float* p) {
+    return (unsigned)(size_t)(__attribute__((address_space(3))) const float*)p;
+}
+
+__device__ __forceinline__ unsigned opaque_u(unsigned v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// Slab of the cross-term form: NPP planes of float pairs, PS pairs each.
+// Pair planes 0 .. BZ/2 hold the MRI planes (2k, 2k+1) interleaved; pair
+// planes BZ/2 + 1 + j hold (L~, E) of CT plane j.
+template <class C>
+struct XtCfg {
+    static_assert(C::APLANES % 2 == 0, "MRI planes are interleaved in pairs");
+    static constexpr int MPAIRS = C::APLANES / 2;
+    static constexpr int NPP = MPAIRS + C::LPLANES;
+    static constexpr int LDS_FLOATS = 2 * NPP * C::PS;
+    static constexpr int LDS_BYTES = LDS_FLOATS * 4;
+    static_assert(LDS_FLOATS >= C::LDS_FLOATS, "the squared-difference slab must fit too");
+    // pair planes per base register: the 16-bit byte immediate of a 64-bit
+    // read also carries the row (up to BX + 1) and dy (up to 2 HW)
+    static constexpr int PPP = (8191 - ((C::BXv + 1) * C::CW + 2 * C::HWv)) / C::PS + 1;
+    static_assert(PPP >= 1, "one pair plane must fit an LDS immediate window");
+    static constexpr int NB = (NPP + PPP - 1) / PPP;
+    // float index of MRI plane pz, and of the (L~, E) pair of CT plane j, at
+    // in-plane position p
+    static __device__ __forceinline__ int mri(int pz, int p) {
+        return 2 * ((pz >> 1) * C::PS + p) + (pz & 1);
+    }
+    static __device__ __forceinline__ int le(int j, int p) { return 2 * ((MPAIRS + j) * C::PS + p); }
+};
+
+// Byte-address base registers of one candidate row: in-plane position `pos`
+// (row, column of the lane's window for dy = 0) in pair planes 0, PPP, ...
+template <class C>
+__device__ __forceinline__ void make_bases_xt(unsigned lds0, int pos,
+                                              unsigned (&b)[XtCfg<C>::NB]) {
+    using X = XtCfg<C>;
+#pragma unroll
+    for (int k = 0; k < X::NB; ++k) b[k] = opaque_u(lds0 + 8u * (unsigned)(pos + k * X::PPP * C::PS));
+}
+
+// The pair at pair plane `pl`, in-plane offset `off` from the bases.
+template <class C>
+__device__ __forceinline__ f32x2 pair_at(const unsigned (&b)[XtCfg<C>::NB], int pl, int off) {
+    using X = XtCfg<C>;
+    const lds_pair_t* q = (const lds_pair_t*)(size_t)b[pl / X::PPP];
+    return q[(pl % X::PPP) * C::PS + off];
+}
+
+// The lane's MRI values of one candidate: a[pz][i] = plane pz, row i.
+template <class C, int BX, int BZ>
+__device__ __forceinline__ void load_mri_xt(const unsigned (&b)[XtCfg<C>::NB],
+                                            float (&a)[BZ + 2][BX + 2]) {
+    using X = XtCfg<C>;
+#pragma unroll
+    for (int k = 0; k < X::MPAIRS; ++k)
+#pragma unroll
+        for (int i = 0; i < BX + 2; ++i) {
+            const f32x2 v = pair_at<C>(b, k, i * C::CW);
+            a[2 * k][i] = v.x;
+            a[2 * k + 1][i] = v.y;
+        }
+}
+
+// One (dx, dy) candidate in cross-term form; t2 holds 2 T'.
+template <class C, int BX, int BZ>
+__device__ __forceinline__ void xterm_step(const unsigned (&b)[XtCfg<C>::NB],
+                                           const float (&t2)[BZ + 2][BX + 2],
+                                           float (&P)[BZ][BX], float (&S)[BZ][BX]) {
+    using X = XtCfg<C>;
+    float a[BZ + 2][BX + 2];
+    load_mri_xt<C, BX, BZ>(b, a);
+    float ex[BZ + 2][BX];
+#pragma unroll
+    for (int pz = 0; pz < BZ + 2; ++pz) {
+#pragma unroll
+        for (int i = 0; i < BX; i += 2) {
+            float q = t2[pz][i + 1] * a[pz][i + 1];
+            q = fmaf(t2[pz][i + 2], a[pz][i + 2], q);
+            ex[pz][i] = fmaf(t2[pz][i], a[pz][i], q);
+            ex[pz][i + 1] = fmaf(t2[pz][i + 3], a[pz][i + 3], q);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < BZ; j += 2) {
+#pragma unroll
+        for (int i = 0; i < BX; ++i) {
+            const float q = ex[j + 1][i] + ex[j + 2][i];
+            const float e0 = ex[j][i] + q;
+            const float e1 = q + ex[j + 3][i];
+            const float c0 = lane_next(e0) + (lane_prev(e0) + e0);
+            const float c1 = lane_next(e1) + (lane_prev(e1) + e1);
+            const float w0 = __builtin_amdgcn_exp2f(c0);
+            const float w1 = __builtin_amdgcn_exp2f(c1);
+            const f32x2 g0 = pair_at<C>(b, X::MPAIRS + j, (i + 1) * C::CW);
+            const f32x2 g1 = pair_at<C>(b, X::MPAIRS + j + 1, (i + 1) * C::CW);
+            P[j][i] = fmaf(w0, g0.x, P[j][i]);
+            S[j][i] = fmaf(w0, g0.y, S[j][i]);
+            P[j + 1][i] = fmaf(w1, g1.x, P[j + 1][i]);
+            S[j + 1][i] = fmaf(w1, g1.y, S[j + 1][i]);
+        }
+    }
+}
+
+// One candidate of a slab the cross-term form cannot take (restaged with
+// the raw CT in the first float of each pair): squared differences
+// (T' = t2 / 2, exact), weight 2^(TT - SSD') on the scale of the cross-term
+// accumulators.
+template <class C, int BX, int BZ>
+__device__ __forceinline__ void scaled_ssd_step(const unsigned (&b)[XtCfg<C>::NB],
+                                                const float (&t2)[BZ + 2][BX + 2],
+                                                const float (&tt)[BZ][BX], float (&P)[BZ][BX],
+                                                float (&S)[BZ][BX]) {
+    using X = XtCfg<C>;
+    float a[BZ + 2][BX + 2];
+    load_mri_xt<C, BX, BZ>(b, a);
+    float ex[BZ + 2][BX];
+#pragma unroll
+    for (int pz = 0; pz < BZ + 2; ++pz) {
+        float d[BX + 2];
+#pragma unroll
+        for (int i = 0; i < BX + 2; ++i) {
+            const float df = fmaf(0.5f, t2[pz][i], -a[pz][i]);
+            d[i] = df * df;
+        }
+#pragma unroll
+        for (int i = 0; i < BX; i += 2) {
+            const float q = d[i + 1] + d[i + 2];
+            ex[pz][i] = d[i] + q;
+            ex[pz][i + 1] = q + d[i + 3];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < BZ; j += 2) {
+#pragma unroll
+        for (int i = 0; i < BX; ++i) {
+            const float q = ex[j + 1][i] + ex[j + 2][i];
+            const float e0 = ex[j][i] + q;
+            const float e1 = q + ex[j + 3][i];
+            const float s0 = lane_next(e0) + (lane_prev(e0) + e0);
+            const float s1 = lane_next(e1) + (lane_prev(e1) + e1);
+            const float w0 = __builtin_amdgcn_exp2f(tt[j][i] - s0);
+            const float w1 = __builtin_amdgcn_exp2f(tt[j + 1][i] - s1);
+            const float l0 = pair_at<C>(b, X::MPAIRS + j, (i + 1) * C::CW).x;
+            const float l1 = pair_at<C>(b, X::MPAIRS + j + 1, (i + 1) * C::CW).x;
+            P[j][i] = fmaf(w0, l0, P[j][i]);
+            S[j][i] += w0;
+            P[j + 1][i] = fmaf(w1, l1, P[j + 1][i]);
+            S[j + 1][i] += w1;
+        }
+    }
+}
+
+// Copies the slab of one (atlas, dz) stage in the cross-term layout: MRI
+// planes z = zdz-1 .. zdz+BZ (scaled, interleaved in pairs) and, for CT
+// planes z = zdz .. zdz+BZ-1, (L * E, E) with E = 2^-AA; positions outside
+// the volume hold zeros.  Returns the lane's flag: a staged MRI value beyond
+// kXtBound or CT value beyond kXtLabelBound (or either not a number).
+template <class C, int NW>
+__device__ __forceinline__ int stage_slab_xt(float* __restrict__ lds, const float* __restrict__ A,
+                                             const float* __restrict__ L,
+                                             const float* __restrict__ AA, float scale, int zdz,
+                                             int xs, int ys, int nx, int ny, int nz,
+                                             long long plane, int wv, int lane) {
+    using X = XtCfg<C>;
+    constexpr int RPW = (C::RX + NW - 1) / NW;   // rows per wave and plane
+    int out_of_range = 0;
+    f32x2* __restrict__ lp = reinterpret_cast<f32x2*>(lds);
+    // MRI: one pair plane (two volume planes) per round
+#pragma unroll 1
+    for (int k = 0; k < X::MPAIRS; ++k) {
+        f32x2 v[RPW][C::COLCHUNKS];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int zz = zdz - 1 + 2 * k + h;
+            const bool zok = zz >= 0 && zz < nz;
+            const long long zoff = (long long)zz * plane;
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int rr = wv + r * NW;
+                const int xx = xs + rr;
+                const bool xok = zok && rr < C::RX && xx >= 0 && xx < nx;
+#pragma unroll
+                for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
+                    const int c = lane + cc * 64;
+                    const int yy = ys + c;
+                    float t = 0.f;
+                    if (xok && c < C::CW && yy >= 0 && yy < ny)
+                        t = A[zoff + (long long)xx * ny + yy] * scale;
+                    out_of_range |= !(fabsf(t) <= kXtBound);
+                    v[r][cc][h] = t;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int rr = wv + r * NW;
+#pragma unroll
+            for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
+                const int c = lane + cc * 64;
+                if (rr < C::RX && c < C::CW) lp[k * C::PS + rr * C::CW + c] = v[r][cc];
+            }
+        }
+    }
+    // CT planes with their E
+#pragma unroll 1
+    for (int j = 0; j < C::LPLANES; ++j) {
+        const int zz = zdz + j;
+        const bool zok = zz >= 0 && zz < nz;
+        const long long zoff = (long long)zz * plane;
+        float lv[RPW][C::COLCHUNKS], av[RPW][C::COLCHUNKS];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int rr = wv + r * NW;
+            const int xx = xs + rr;
+            const bool xok = zok && rr < C::RX && xx >= 0 && xx < nx;
+#pragma unroll
+            for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
+                const int c = lane + cc * 64;
+                const int yy = ys + c;
+                const bool in = xok && c < C::CW && yy >= 0 && yy < ny;
+                const long long o = zoff + (long long)xx * ny + yy;
+                lv[r][cc] = in ? L[o] : 0.f;
+                av[r][cc] = in ? AA[o] : __builtin_inff();
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int rr = wv + r * NW;
+#pragma unroll
+            for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
+                const int c = lane + cc * 64;
+                out_of_range |= !(fabsf(lv[r][cc]) <= kXtLabelBound);
+                const float e = __builtin_amdgcn_exp2f(-av[r][cc]);   // 0 outside the volume
+                if (rr < C::RX && c < C::CW)
+                    lp[(X::MPAIRS + j) * C::PS + rr * C::CW + c] = f32x2{lv[r][cc] * e, e};
+            }
+        }
+    }
+    return out_of_range;
+}
+
+// Rewrites the first float of every (L~, E) pair of the slab with the raw CT
+// (zero outside the volume), for the squared-difference weighing of a slab
+// out of the cross-term range.
+template <class C, int NW>
+__device__ __forceinline__ void restage_ct_raw(float* __restrict__ lds,
+                                               const float* __restrict__ L, int zdz, int xs,
+                                               int ys, int nx, int ny, int nz, long long plane,
+                                               int wv, int lane) {
+    using X = XtCfg<C>;
+    constexpr int RPW = (C::RX + NW - 1) / NW;
+#pragma unroll 1
+    for (int j = 0; j < C::LPLANES; ++j) {
+        const int zz = zdz + j;
+        const bool zok = zz >= 0 && zz < nz;
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int rr = wv + r * NW;
+            const int xx = xs + rr;
+            const bool xok = zok && rr < C::RX && xx >= 0 && xx < nx;
+#pragma unroll
+            for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
+                const int c = lane + cc * 64;
+                const int yy = ys + c;
+                if (rr < C::RX && c < C::CW)
+                    lds[X::le(j, rr * C::CW + c)] =
+                        (xok && yy >= 0 && yy < ny) ? L[(long long)zz * plane + (long long)xx * ny + yy]
+                                                    : 0.f;
+            }
+        }
+    }
+}
+
+// One work item in cross-term form where the value range allows it (see
+// above); the same partial sums as tiled_item within the FP32 tolerance.
+template <int W, int BX, int BZ, int NW, int HW>
+__device__ __forceinline__ void tiled_item_xt(const TiledArgs& args, int item,
+                                              float* __restrict__ lds) {
+    using C = TiledCfg<W, BX, BZ, NW, HW>;
+    using X = XtCfg<C>;
+    const int g = item / args.n_tiles;
+    int t = item - g * args.n_tiles;
+    const int ty = t % args.tiles_y;
+    t /= args.tiles_y;
+    const int tx = t % args.tiles_x;
+    const int tz = t / args.tiles_x;
+
+    const int tid = threadIdx.x;
+    const int wv = tid >> 6;
+    const int lane = tid & 63;
+    const int seg = lane / W;
+    const int ly = lane % W;
+    const int sg = wv * C::S + seg;
+
+    const int nx = args.nx, ny = args.ny, nz = args.nz, r = args.r;
+    const long long plane = (long long)nx * ny;
+    const int y0 = r + ty * C::OY;
+    const int x0 = r + tx * C::WX;
+    const int z0 = r + tz * BZ;
+    const int y = y0 - 1 + ly;
+    const int xb = x0 + sg * BX;
+
+    float tv[BZ + 2][BX + 2];
+    load_target_block<BX, BZ>(args.img, args.scale, z0, xb, y, nx, ny, nz, plane, tv);
+    int big = 0;
+#pragma unroll
+    for (int pz = 0; pz < BZ + 2; ++pz)
+#pragma unroll
+        for (int i = 0; i < BX + 2; ++i) big |= !(fabsf(tv[pz][i]) <= kXtBound);
+    const bool squared_item = __syncthreads_or(big) != 0;
+
+    float P[BZ][BX], S[BZ][BX];
+#pragma unroll
+    for (int j = 0; j < BZ; ++j)
+#pragma unroll
+        for (int i = 0; i < BX; ++i) P[j][i] = S[j][i] = 0.f;
+
+    const int lane_base = sg * BX * C::CW + ly;
+    const int xs = x0 - 1 - HW;
+    const int ys = y0 - 1 - HW;
+    const int a_begin = g * args.group;
+    const int a_end = min(args.n_atlas, a_begin + args.group);
+    // 2^-TT per output (1 where the item keeps the squared differences)
+    float f[BZ][BX];
+
+    if (squared_item) {
+        for (int a = a_begin; a < a_end; ++a) {
+            const float* __restrict__ A = args.atlas + (long long)a * args.N;
+            const float* __restrict__ L = args.label + (long long)a * args.N;
+            for (int dz = -HW; dz <= HW; ++dz) {
+                __syncthreads();
+                stage_slab<C, NW>(lds, A, L, args.scale, z0 + dz, xs, ys, nx, ny, nz, plane, wv,
+                                  lane);
+                __syncthreads();
+#pragma unroll 1
+                for (int dx = 0; dx <= 2 * HW; ++dx) {
+#pragma unroll 1
+                    for (int dy = 0; dy <= 2 * HW; ++dy)
+                        tiled_step<C, BX, BZ>(lds, lane_base + dx * C::CW + dy, tv, P, S);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < BZ; ++j)
+#pragma unroll
+            for (int i = 0; i < BX; ++i) f[j][i] = 1.f;
+    } else {
+        // TT per output: 3x3 sums of T'^2 in (z, x), then over y across lanes
+        float tt[BZ][BX];
+#pragma unroll
+        for (int j = 0; j < BZ; ++j)
+#pragma unroll
+            for (int i = 0; i < BX; ++i) {
+                float s = 0.f;
+#pragma unroll
+                for (int pz = j; pz < j + 3; ++pz)
+#pragma unroll
+                    for (int px = i; px < i + 3; ++px) s = fmaf(tv[pz][px], tv[pz][px], s);
+                tt[j][i] = lane_next(s) + (lane_prev(s) + s);
+            }
+#pragma unroll
+        for (int pz = 0; pz < BZ + 2; ++pz)
+#pragma unroll
+            for (int i = 0; i < BX + 2; ++i) tv[pz][i] *= 2.f;
+        const unsigned lds0 = lds_byte_address(lds);
+        for (int a = a_begin; a < a_end; ++a) {
+            const float* __restrict__ A = args.atlas + (long long)a * args.N;
+            const float* __restrict__ L = args.label + (long long)a * args.N;
+            const float* __restrict__ AA = args.aa + (long long)a * args.N;
+            for (int dz = -HW; dz <= HW; ++dz) {
+                __syncthreads();
+                const int flag = stage_slab_xt<C, NW>(lds, A, L, AA, args.scale, z0 + dz, xs, ys,
+                                                      nx, ny, nz, plane, wv, lane);
+                if (__syncthreads_or(flag) == 0) {
+#pragma unroll 1
+                    for (int dx = 0; dx <= 2 * HW; ++dx) {
+                        unsigned b[X::NB];
+                        make_bases_xt<C>(lds0, lane_base + dx * C::CW, b);
+#pragma unroll 1
+                        for (int dy = 0; dy <= 2 * HW; ++dy) {
+                            unsigned bb[X::NB];
+#pragma unroll
+                            for (int k = 0; k < X::NB; ++k) bb[k] = b[k] + 8u * (unsigned)dy;
+                            xterm_step<C, BX, BZ>(bb, tv, P, S);
+                        }
+                    }
+                } else {
+                    restage_ct_raw<C, NW>(lds, L, z0 + dz, xs, ys, nx, ny, nz, plane, wv, lane);
+                    __syncthreads();
+#pragma unroll 1
+                    for (int dx = 0; dx <= 2 * HW; ++dx) {
+#pragma unroll 1
+                        for (int dy = 0; dy <= 2 * HW; ++dy) {
+                            unsigned b[X::NB];
+                            make_bases_xt<C>(lds0, lane_base + dx * C::CW + dy, b);
+                            scaled_ssd_step<C, BX, BZ>(b, tv, tt, P, S);
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < BZ; ++j)
+#pragma unroll
+            for (int i = 0; i < BX; ++i) f[j][i] = __builtin_amdgcn_exp2f(-tt[j][i]);
+    }
+
+    if (ly >= 1 && ly <= W - 2 && y < ny - r) {
+        float* __restrict__ wsP = args.ws + (long long)(2 * g) * args.N;
+        float* __restrict__ wsS = wsP + args.N;
+#pragma unroll
+        for (int j = 0; j < BZ; ++j)
+#pragma unroll
+            for (int i = 0; i < BX; ++i) {
+                const int z = z0 + j, x = xb + i;
+                if (z < nz - r && x < nx - r) {
+                    const long long idx = z * plane + (long long)x * ny + y;
+                    wsP[idx] = P[j][i] * f[j][i];
+                    wsS[idx] = S[j][i] * f[j][i];
+                }
+            }
+    }
+}
+
+template <int W, int BX, int BZ, int NW, int HW>
+__global__ __launch_bounds__(NW * 64) void fuse_tiled_xt(TiledArgs args) {
+    extern __shared__ __attribute__((aligned(16))) float lds[];
+    const int item = xcd_contiguous(blockIdx.x, args.n_items);
+    if (args.only && !args.only[item]) return;   // uniform over the workgroup
+    tiled_item_xt<W, BX, BZ, NW, HW>(args, item, lds);
+}
+
+// ---------------------------------------------------------------------------
 // host helpers shared by pct_fuse.hip and pct_batch.hip (defined in pct_fuse.hip)
 // ---------------------------------------------------------------------------
 
@@ -442,8 +922,23 @@ int check_device(int* dev_out);
 // The library's per-device workspace, grown to at least `bytes`.
 int ensure_workspace(int dev, size_t bytes, void** ptr);
 
-// Bytes of partial sums a tiled run with `n_groups` atlas groups needs.
-size_t tiled_ws_bytes(const pct_problem* pb, int n_groups);
+// Bytes of workspace a tiled run of `shape_id` with `n_groups` atlas groups
+// needs: the partial sums, then for a cross-term shape the AA of every atlas
+// voxel (at tiled_aa_offset).
+size_t tiled_ws_bytes(const pct_problem* pb, int n_groups, int shape_id);
+size_t tiled_aa_offset(const pct_problem* pb, int n_groups);
+
+// Bytes of workspace the device post chain needs for a volume, and the chain
+// itself on a caller's workspace of at least that size (pct_post.hip).
+size_t postprocess_ws_bytes(int nh, int nx, int ny, int nz);
+int postprocess_dev_ws(int nh, int nx, int ny, int nz, float* d_arr, void* ws, hipStream_t s);
+
+// Whether `shape_id` is a cross-term shape (its kernel reads TiledArgs::aa).
+bool is_xterm_shape(int shape_id);
+
+// Enqueues atlas_sq_box for atlases [0, n_atlas) of `atlas` into `aa`.
+int launch_atlas_aa(const pct_problem* pb, const float* atlas, int n_atlas, float scale,
+                    float* aa, hipStream_t stream);
 
 TiledArgs make_tiled_args(const pct_problem* pb, const TileGeometry& g, const float* img,
                           const float* atlas, const float* label, float* ws);

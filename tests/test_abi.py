@@ -39,7 +39,7 @@ def test_reference_and_oracle_libraries_load():
 
 def test_version_and_strerror():
     L = _lib.lib()
-    assert L.pct_abi_version() == 1
+    assert L.pct_abi_version() == 2
     for code in range(0, -10, -1):
         assert L.pct_strerror(code)
     assert L.pct_strerror(-99) == b"unknown status"

@@ -21,7 +21,7 @@ import pytest
 import oracle
 import pseudoctimaging_amd as pct
 from pseudoctimaging_amd import _lib
-from parity import compare
+from parity import MAX_REL, compare
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -127,7 +127,15 @@ def test_cfg3_constant_mri_is_window_mean(gpu):
                 box += (-1) ** (3 - mz - mx - my) * c[sl]
     want = box / (n * k ** 3)
     inner = got[r:nz - r, r:nx - r, r:ny - r].astype(np.float64)
-    assert np.abs(inner - want).max() < 1e-3
+    # each work item sums K = 1331 float terms: the bar is the plain path's
+    # relative tolerance (at a window mean of about 128 label units that is
+    # 0.013 label units, 0.2 HU).  Every weight here is the same value, so
+    # the rounding of a float running sum is the same at every add; its bias
+    # is at most sum_k ulp(k c) / 2 / (K c) = 2^-25 K = 4e-5 relative, the
+    # bound on the mean error
+    rel = np.abs(inner - want) / np.abs(want)
+    assert rel.max() < MAX_REL, rel.max()
+    assert rel.mean() < 2.0 ** -25 * 1331, rel.mean()
 
 
 def test_cfg1_full_volume(gpu):
