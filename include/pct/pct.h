@@ -131,6 +131,21 @@ int pct_fuse_dev_ws(const pct_problem* pb, const float* d_img, const float* d_at
                     const float* d_atlas_label_norm, float* d_out, int choice,
                     void* d_workspace, size_t workspace_bytes, pct_stream_t stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): pct_fuse_dev with no allocation of any kind.  Its partial
+ * sums live in a 64 MiB array that the library's code object holds on every
+ * GPU it runs on until the process ends; it runs one atlas group with the
+ * squared-difference kernels, so the tiled choices need 8 * nx*ny*nz bytes
+ * of it (at most 8 388 608 voxels, e.g. 256x256x128; larger volumes return
+ * PCT_ERR_TOO_LARGE) and DIRECT needs none.  Output as pct_fuse_dev's within
+ * the FP32 tolerance (another tile shape and atlas grouping).  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other threads) race and must be ordered by the caller. */
+int pct_fuse_dev_noalloc(const pct_problem* pb, const float* d_img, const float* d_atlas,
+                         const float* d_atlas_label_norm, float* d_out, int choice,
+                         pct_stream_t stream);
+
 /* ---- kernel configuration / autotuner ---- */
 
 /* Description of the tiled (SM) kernel configuration chosen for a problem. */

@@ -100,6 +100,8 @@ SIGNATURES = {
     "pct_fuse_dev": (c_int, [POINTER(Problem), c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                              c_void_p]),
     "pct_fuse_workspace_size": (c_size_t, [POINTER(Problem), c_int]),
+    "pct_fuse_dev_noalloc": (c_int, [POINTER(Problem), c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_int, c_void_p]),
     "pct_fuse_dev_ws": (c_int, [POINTER(Problem), c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                 c_void_p, c_size_t, c_void_p]),
     "pct_tile_config_count": (c_int, []),

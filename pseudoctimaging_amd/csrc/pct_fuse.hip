@@ -774,6 +774,91 @@ extern "C" int pct_fuse_dev_ws(const pct_problem* pb, const float* d_img, const 
                          reinterpret_cast<hipStream_t>(stream));
 }
 
+// ---- allocation-free variant of pct_fuse_dev --------------------------------
+//
+// The partial sums live in `scratch`, a file-scope device array of fixed size
+// that every GPU holds from module load to process end.  Its address on a
+// device comes from hipGetSymbolAddress (the array's name in host code is
+// only a stand-in), fetched once per device with that device current.  The
+// run uses one atlas group (every item sums all atlases, so the partials are
+// 2 N floats and every interior voxel is written by exactly one item before
+// the finalize pass reads it: nothing needs clearing) and the
+// squared-difference kernels, which need no AA workspace.
+
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mutex;
+static std::map<int, void*> g_scratch_ptr;   // device -> address of `scratch`
+
+static int scratch_address(int dev, void** ptr) {
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+        void* p = nullptr;
+        // `dev` is current: check_device read it with hipGetDevice
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return PCT_ERR_HIP;
+        }
+        it = g_scratch_ptr.emplace(dev, p).first;
+    }
+    *ptr = it->second;
+    return PCT_OK;
+}
+
+// The squared-difference shape with one atlas group: the window's measured
+// squared-difference preference where it fits, else the cost model's choice
+// among squared-difference shapes.
+static bool choose_noalloc(const pct_problem* pb, int* shape_id, TileGeometry* geo) {
+    static const int kSquaredPref[kMaxTiledHalfWindow + 1] = {-1, -1, -1, 11, 9, 7, -1, -1};
+    const int hw = (pb->nh - 1) / 2;
+    const int pref = hw >= 1 && hw <= kMaxTiledHalfWindow ? kSquaredPref[hw] : -1;
+    if (pref >= 0 && make_geometry(pb, pref, pb->n_atlas, geo)) {
+        *shape_id = pref;
+        return true;
+    }
+    double best = -1;
+    for (int sid = 0; sid < (int)shapes().size(); ++sid) {
+        if (is_xterm_shape(sid)) continue;
+        TileGeometry g;
+        const double c = model_cost(pb, sid, pb->n_atlas, &g);
+        if (c >= 0 && (best < 0 || c < best * 0.999)) {
+            best = c;
+            *shape_id = sid;
+            *geo = g;
+        }
+    }
+    return best >= 0;
+}
+
+extern "C" int pct_fuse_dev_noalloc(const pct_problem* pb, const float* d_img, const float* d_atlas,
+                                    const float* d_label, float* d_out, int choice,
+                                    pct_stream_t stream) {
+    int rc = validate_problem(pb);
+    if (rc) return rc;
+    if (!d_img || !d_atlas || !d_label || !d_out) return PCT_ERR_INVALID;
+    if (!valid_choice(choice)) return PCT_ERR_INVALID;
+    int dev = 0;
+    rc = check_device(&dev);
+    if (rc) return rc;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (!is_tiled_choice(pb, choice))   // the direct kernel uses no scratch
+        return launch_direct(pb, d_img, d_atlas, d_label, d_out, s);
+    int sid = -1;
+    TileGeometry g;
+    if (!choose_noalloc(pb, &sid, &g)) return PCT_ERR_UNSUPPORTED;
+    const size_t need = tiled_ws_bytes(pb, g.n_groups, sid);
+    if (need > kNoAllocScratchBytes) return PCT_ERR_TOO_LARGE;
+    void* ws = nullptr;
+    rc = scratch_address(dev, &ws);
+    if (rc) return rc;
+    TiledArgs a = make_tiled_args(pb, g, d_img, d_atlas, d_label, static_cast<float*>(ws));
+    rc = launch_tiled_items(a, sid, g.hw, s);
+    if (rc) return rc;
+    return launch_finalize(pb, g, static_cast<const float*>(ws), d_out, s);
+}
+
 // Device buffers, streams and events of the host-buffer entry points, kept
 // per device between calls (grown on demand, freed by pct_release).  One
 // host-buffer call at a time runs on a device.
