@@ -85,6 +85,32 @@ __global__ __launch_bounds__(NW * 64) void fuse_batched(BatchArgs args) {
     }
 }
 
+// fuse_batched with the cross-term items (TiledArgs::aa filled per subject).
+template <int W, int BX, int BZ, int NW, int HW>
+__global__ __launch_bounds__(NW * 64) void fuse_batched_xt(BatchArgs args) {
+    extern __shared__ __attribute__((aligned(16))) float lds[];
+    __shared__ long long claimed;
+    const int home = blockIdx.x & (kChunks - 1);
+    for (int c = 0; c < kChunks; ++c) {
+        const int x = (home + c) & (kChunks - 1);
+        for (;;) {
+            if (threadIdx.x == 0) claimed = (long long)atomicAdd(args.counters + x, 1ULL);
+            __syncthreads();
+            long long pos = claimed;
+            __syncthreads();
+            int k = 0;
+            for (; k < args.n_sub; ++k) {
+                const long long len = part_len(args.n_items[k], x);
+                if (pos < len) break;
+                pos -= len;
+            }
+            if (k == args.n_sub) break;   // chunk x is drained: every later claim is too
+            tiled_item_xt<W, BX, BZ, NW, HW>(
+                args.sub[k], (int)(part_start(args.n_items[k], x) + pos), lds);
+        }
+    }
+}
+
 // Written as a kernel rather than a memset so that it can sit in a captured
 // graph.
 __global__ void reset_queues(unsigned long long* counters, int n) {
@@ -107,6 +133,15 @@ BatchKernel make_batch_kernel(int shape_id) {
                         &fuse_batched<W, BX, BZ, NW, 7>}};
 }
 
+template <int W, int BX, int BZ, int NW>
+BatchKernel make_batch_kernel_xt(int shape_id) {
+    return BatchKernel{shape_id,
+                       {&fuse_batched_xt<W, BX, BZ, NW, 1>, &fuse_batched_xt<W, BX, BZ, NW, 2>,
+                        &fuse_batched_xt<W, BX, BZ, NW, 3>, &fuse_batched_xt<W, BX, BZ, NW, 4>,
+                        &fuse_batched_xt<W, BX, BZ, NW, 5>, &fuse_batched_xt<W, BX, BZ, NW, 6>,
+                        &fuse_batched_xt<W, BX, BZ, NW, 7>}};
+}
+
 // Tile shapes with persistent instantiations (ids of shapes()).
 static const std::vector<BatchKernel>& batch_kernels() {
     static const std::vector<BatchKernel> k = {
@@ -116,6 +151,8 @@ static const std::vector<BatchKernel>& batch_kernels() {
         make_batch_kernel<32, 2, 2, 8>(10),
         make_batch_kernel<32, 2, 2, 4>(11),
         make_batch_kernel<32, 2, 4, 8>(13),
+        make_batch_kernel_xt<32, 2, 2, 4>(16),
+        make_batch_kernel_xt<32, 2, 2, 8>(17),
     };
     return k;
 }
@@ -239,6 +276,21 @@ static int enqueue_batch(const pct_subject* s, const BatchPlan& p, const std::ve
     const bool concurrent = !p.choice.serial && nc > 1;
     hipLaunchKernelGGL(reset_queues, dim3(1), dim3(64), 0, stream, counters, nc * kChunks);
     if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
+    // AA of the atlases of every subject of a cross-term class, before the
+    // class kernels (and the fork of concurrent classes)
+    for (int c = 0; c < nc; ++c) {
+        if (!is_xterm_shape(p.choice.shape_id[p.class_hw[c]])) continue;
+        for (int i : p.classes[c]) {
+            const TiledArgs t = make_tiled_args(&s[i].pb, p.geo[i], s[i].d_img, s[i].d_atlas,
+                                                s[i].d_atlas_label_norm, nullptr);
+            const int rc = launch_atlas_aa(
+                &s[i].pb, s[i].d_atlas, s[i].pb.n_atlas, t.scale,
+                reinterpret_cast<float*>(base + p.ws_off[i] +
+                                         tiled_aa_offset(&s[i].pb, p.geo[i].n_groups)),
+                stream);
+            if (rc) return rc;
+        }
+    }
     if (concurrent && hipEventRecord(events[0], stream) != hipSuccess) return PCT_ERR_HIP;
     for (int c = 0; c < nc; ++c) {
         const int hw = p.class_hw[c];
@@ -256,6 +308,9 @@ static int enqueue_batch(const pct_subject* s, const BatchPlan& p, const std::ve
             a.sub[k] = make_tiled_args(&s[i].pb, p.geo[i], s[i].d_img, s[i].d_atlas,
                                        s[i].d_atlas_label_norm,
                                        reinterpret_cast<float*>(base + p.ws_off[i]));
+            if (is_xterm_shape(sid))
+                a.sub[k].aa = reinterpret_cast<const float*>(
+                    base + p.ws_off[i] + tiled_aa_offset(&s[i].pb, p.geo[i].n_groups));
             a.n_items[k] = p.geo[i].n_items;
             ++k;
         }
