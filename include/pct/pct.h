@@ -70,6 +70,9 @@ typedef struct pct_problem {
     int patch;        /* patch edge; must be 3 (cpp:1240) */
     double sigma;     /* cpp:1560 */
     double beta;      /* cpp:1561 */
+    double div;       /* > 0: the weight divisor itself, as the reference's
+                         wrappers take it (cu:1635, cpp:1350), and sigma, beta
+                         are not used; 0: div = pct_weight_divisor(sigma, beta) */
 } pct_problem;
 
 const char* pct_strerror(int status);

@@ -35,7 +35,8 @@ CHOICE_GPU_TEST_TIME = 4
 class Problem(Structure):
     """pct_problem"""
     _fields_ = [("nx", c_int), ("ny", c_int), ("nz", c_int), ("n_atlas", c_int),
-                ("nh", c_int), ("patch", c_int), ("sigma", c_double), ("beta", c_double)]
+                ("nh", c_int), ("patch", c_int), ("sigma", c_double), ("beta", c_double),
+                ("div", c_double)]
 
 
 class TileConfig(Structure):

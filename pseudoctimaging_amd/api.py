@@ -41,11 +41,14 @@ def _f32(a, name: str) -> np.ndarray:
     return a
 
 
-def problem(shape, n_atlas: int, nh: int, sigma: float = 1.0, beta: float = 1.0) -> Problem:
-    """pct_problem for a volume of shape (nz, nx, ny)."""
+def problem(shape, n_atlas: int, nh: int, sigma: float = 1.0, beta: float = 1.0,
+            div: float = 0.0) -> Problem:
+    """pct_problem for a volume of shape (nz, nx, ny).  div > 0 gives the
+    weight divisor itself (the `div` argument of the reference's wrappers,
+    cu:1635 / cpp:1350), and sigma and beta are then not used."""
     nz, nx, ny = (int(s) for s in shape)
     return Problem(nx=nx, ny=ny, nz=nz, n_atlas=int(n_atlas), nh=int(nh), patch=3,
-                   sigma=float(sigma), beta=float(beta))
+                   sigma=float(sigma), beta=float(beta), div=float(div))
 
 
 def weight_divisor(sigma: float, beta: float) -> float:
@@ -72,12 +75,14 @@ def device_count() -> int:
 
 
 def multi_patch_based_segmentation_gpu(img, atlas, atlas_label, nh: int, sigma: float = 1.0,
-                                       beta: float = 1.0, choice: int = CHOICE_SM) -> np.ndarray:
+                                       beta: float = 1.0, choice: int = CHOICE_SM,
+                                       div: float = 0.0) -> np.ndarray:
     """Fused value of every voxel (interior: P/S or NaN, border: 0).
 
     `atlas_label` is the normalised CT (see normalize_labels).  Host arrays in
-    and out; device memory is allocated and released per call like the
-    reference wrapper (cu:1687-1744).
+    and out, as the reference wrapper (cu:1687-1744); the library keeps its
+    device buffers for the next call.  div > 0 is the reference's `div`
+    argument (cu:1635) and replaces sigma and beta.
     """
     img = _f32(img, "img")
     atlas = _f32(atlas, "atlas")
@@ -85,7 +90,7 @@ def multi_patch_based_segmentation_gpu(img, atlas, atlas_label, nh: int, sigma: 
     if img.ndim != 3 or atlas.ndim != 4 or atlas.shape != atlas_label.shape \
             or atlas.shape[1:] != img.shape:
         raise ValueError("expected img (nz,nx,ny) and atlas/atlas_label (n,nz,nx,ny)")
-    pb = problem(img.shape, atlas.shape[0], nh, sigma, beta)
+    pb = problem(img.shape, atlas.shape[0], nh, sigma, beta, div)
     out = np.zeros(img.shape, np.float32)
     check(lib().pct_fuse_host(byref(pb), fptr(img), fptr(atlas), fptr(atlas_label), fptr(out),
                               int(choice)), "pct_fuse_host")
@@ -143,14 +148,15 @@ def check_volume_set(img, atlas, atlas_label, out=None, stream=None) -> None:
 
 
 def fuse_device(img, atlas, atlas_label, out, nh: int, sigma: float = 1.0, beta: float = 1.0,
-                choice: int = CHOICE_SM, stream=None) -> None:
+                choice: int = CHOICE_SM, stream=None, div: float = 0.0) -> None:
     """Fusion on tensors already resident in HBM (torch only carries memory
-    and the stream).  Enqueued on `stream` (default: torch's current one)."""
+    and the stream).  Enqueued on `stream` (default: torch's current one).
+    div > 0 replaces sigma and beta, as in multi_patch_based_segmentation_gpu."""
     import torch
     check_volume_set(img, atlas, atlas_label, out, stream)
     if stream is None:
         stream = torch.cuda.current_stream(img.device)
-    pb = problem(img.shape, atlas.shape[0], nh, sigma, beta)
+    pb = problem(img.shape, atlas.shape[0], nh, sigma, beta, div)
     check(lib().pct_fuse_dev(byref(pb), _dev_ptr(img, "img"), _dev_ptr(atlas, "atlas"),
                              _dev_ptr(atlas_label, "atlas_label"), _dev_ptr(out, "out"),
                              int(choice), ctypes.c_void_p(stream.cuda_stream)), "pct_fuse_dev")

@@ -269,12 +269,19 @@ int validate_problem(const pct_problem* pb) {
     if (!pb) return PCT_ERR_INVALID;
     if (pb->nx < 1 || pb->ny < 1 || pb->nz < 1 || pb->n_atlas < 1) return PCT_ERR_INVALID;
     if (pb->nh < 1 || pb->patch != 3) return PCT_ERR_INVALID;
-    if (!(pb->sigma > 0.0) || !(pb->beta > 0.0)) return PCT_ERR_INVALID;
-    if (!std::isfinite(pb->sigma) || !std::isfinite(pb->beta)) return PCT_ERR_INVALID;
+    if (!(pb->div >= 0.0) || !std::isfinite(pb->div)) return PCT_ERR_INVALID;
+    if (pb->div == 0.0) {
+        if (!(pb->sigma > 0.0) || !(pb->beta > 0.0)) return PCT_ERR_INVALID;
+        if (!std::isfinite(pb->sigma) || !std::isfinite(pb->beta)) return PCT_ERR_INVALID;
+    }
     const long long N = (long long)pb->nx * pb->ny * pb->nz;
     if (pb->nx > 65535 || pb->ny > 65535 || pb->nz > 65535) return PCT_ERR_TOO_LARGE;
     if (N * (long long)pb->n_atlas > (1LL << 40)) return PCT_ERR_TOO_LARGE;
     return PCT_OK;
+}
+
+double problem_divisor(const pct_problem* pb) {
+    return pb->div > 0.0 ? pb->div : pct_weight_divisor(pb->sigma, pb->beta);
 }
 
 static int interior(int n, int r) { return std::max(0, n - 2 * r); }
@@ -569,7 +576,7 @@ TiledArgs make_tiled_args(const pct_problem* pb, const TileGeometry& g, const fl
     a.n_items = (int)g.n_items;
     a.n_atlas = pb->n_atlas;
     a.group = g.group;
-    const double div = pct_weight_divisor(pb->sigma, pb->beta);
+    const double div = problem_divisor(pb);
     a.scale = (float)std::sqrt(1.4426950408889634 / div);
     a.only = nullptr;
     a.aa = nullptr;
@@ -656,7 +663,7 @@ int launch_direct(const pct_problem* pb, const float* img, const float* atlas,
                          const float* label, float* out, hipStream_t stream) {
     const long long N = (long long)pb->nx * pb->ny * pb->nz;
     const int hw = (pb->nh - 1) / 2;
-    const double div = pct_weight_divisor(pb->sigma, pb->beta);
+    const double div = problem_divisor(pb);
     const float k = (float)(1.4426950408889634 / div);
     const dim3 block(256), grid((pb->ny + 255) / 256, pb->nx, pb->nz);
     TimedLaunch timed(stream);

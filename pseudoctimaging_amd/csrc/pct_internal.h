@@ -34,6 +34,10 @@ struct TileGeometry {
 // Problem validation shared by every entry point.
 int validate_problem(const pct_problem* pb);
 
+// The weight divisor of a problem: pb->div when given, else
+// pct_weight_divisor(sigma, beta).
+double problem_divisor(const pct_problem* pb);
+
 // Largest element count a reader accepts for all of a file's volumes together.
 constexpr long long kMaxFileElements = 1LL << 34;
 

@@ -226,7 +226,7 @@ static size_t fill_ws_bytes(long long interior) {
 
 static int enqueue_fill(int nh, int nx, int ny, int nz, float* d_arr, void* ws, hipStream_t s) {
     const int lo = (nh - 1) / 2 + 1;
-    const long long iz = nz - 2 * lo, ix = nx - 2 * lo, iy = ny - 2 * lo;
+    const long long ix = nx - 2 * lo, iy = ny - 2 * lo;
     const long long interior = fill_interior(nh, nx, ny, nz);
     if (interior == 0) return PCT_OK;
     const size_t list_bytes = ((size_t)interior * 4 + 255) & ~(size_t)255;

@@ -434,7 +434,7 @@ static TrainArgs make_train_args(const pct_problem* pb, const TrainPlan& t, cons
     a.loss = reinterpret_cast<float*>(w + t.o_loss);
     a.mu = params[kOffMu];
     a.inv_sd = params[kOffSd];
-    a.teacher_t = teacher_t > 0.f ? teacher_t : (float)pct_weight_divisor(pb->sigma, pb->beta);
+    a.teacher_t = teacher_t > 0.f ? teacher_t : (float)problem_divisor(pb);
     a.tau = tau;
     return a;
 }

@@ -1,0 +1,116 @@
+"""The batched and learned configurations at their full benchmark sizes.
+
+* Six uneven subjects in one batched launch: 160x160x96 n=1 NH=7,
+  192x192x96 n=3 NH=9, 222x222x112 n=5 and n=7 NH=11, 256x256x128 n=10 NH=9,
+  128x128x64 n=10 NH=7.  Each subject equals its individual run with its
+  window class's tile shape and atlas group bit for bit, and sampled voxels
+  match the FP64 oracle within the plain path's tolerance.
+* The learned-pruned path at 222x222x112, n=10, NH=11 (the inputs of the
+  cfg3 golden fixture, whose samples the plain leg is checked against), the
+  embedding trained on 4 other seeded phantoms: mean |dHU| against the plain
+  output at most 5 HU, the NaN pattern identical, the fallback fraction a
+  fraction.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import pseudoctimaging_amd as pct
+from parity import compare
+from pseudoctimaging_amd import learned
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+BATCH = [(160, 160, 96, 1, 7), (192, 192, 96, 3, 9), (222, 222, 112, 5, 11),
+         (222, 222, 112, 7, 11), (256, 256, 128, 10, 9), (128, 128, 64, 10, 7)]
+
+
+def test_batch_baseline_cfg4(gpu):
+    import torch
+    dev = torch.device("cuda:0")
+    host, subs = [], []
+    for k, (nx, ny, nz, n, nh) in enumerate(BATCH):
+        print(f"case cfg4 subject {k}: {nx}x{ny}x{nz} n={n} NH={nh}", flush=True)
+        img, amri, act = pct.synth_tissue(nx, ny, nz, n, 3, False, 500 + k)
+        lab = act.copy()
+        pct.normalize_labels(lab)
+        host.append((img, amri, lab, nh))
+        subs.append({"img": torch.from_numpy(img).to(dev), "atlas": torch.from_numpy(amri).to(dev),
+                     "label": torch.from_numpy(lab).to(dev),
+                     "out": torch.full(img.shape, -7.0, device=dev), "nh": nh,
+                     "sigma": 2.5, "beta": 1.0})
+    print("batched launch", flush=True)
+    pct.fuse_batch(subs)
+    torch.cuda.synchronize()
+    classes = pct.batch_choice(subs)["classes"]
+    rng = np.random.default_rng(11)
+    try:
+        for k, ((img, amri, lab, nh), s) in enumerate(zip(host, subs)):
+            sid, grp = classes[(nh - 1) // 2]
+            print(f"subject {k}: shape {sid} group {grp}", flush=True)
+            got = s["out"].cpu().numpy()
+            pct.force_tile_config(sid, grp)
+            single = torch.full(img.shape, -5.0, device=dev)
+            pct.fuse_device(s["img"], s["atlas"], s["label"], single, nh, 2.5, 1.0)
+            torch.cuda.synchronize()
+            assert np.array_equal(got.view(np.uint32), single.cpu().numpy().view(np.uint32)), k
+            nz, nx, ny = img.shape
+            r = (nh - 1) // 2 + 1
+            zi, xi, yi = (rng.integers(r, d - r, 500) for d in (nz, nx, ny))
+            flat = (zi * nx + xi) * ny + yi
+            ref = oracle.voxels(img, amri, lab, nh, 2.5, 1.0, flat)
+            print(compare(got.reshape(-1)[flat], ref), flush=True)
+            border = np.ones(img.shape, bool)
+            border[r:nz - r, r:nx - r, r:ny - r] = False
+            assert not got[border].any()
+    finally:
+        pct.force_tile_config(-1, 0)
+
+
+def test_learned_baseline_cfg5(gpu):
+    import torch
+    g = dict(np.load(os.path.join(GOLDEN, "cfg3_sample.npz")))
+    nx, ny, nz, n, nh = (int(v) for v in g["params"][:5])
+    sigma, beta, seed, shift = (float(g["params"][5]), float(g["params"][6]),
+                                int(g["params"][7]), int(g["params"][8]))
+    assert (nx, ny, nz, n, nh) == (222, 222, 112, 10, 11)
+    print(f"case cfg5: {nx}x{ny}x{nz} n={n} NH={nh}", flush=True)
+    dev = torch.device("cuda:0")
+    img, amri, act = pct.synth_tissue(nx, ny, nz, n, shift, False, seed)
+    lab = act.copy()
+    pct.normalize_labels(lab)
+    d_img, d_amri, d_lab = (torch.from_numpy(x).to(dev) for x in (img, amri, lab))
+    plain = torch.empty(img.shape, device=dev)
+    pct.fuse_device(d_img, d_amri, d_lab, plain, nh, sigma, beta)
+    torch.cuda.synchronize()
+    p = plain.cpu().numpy()
+    # the plain leg against the compiled reference's samples
+    print(compare(p.reshape(-1)[g["index"]], g["ref_fused"]), flush=True)
+    # the embedding, trained on 4 other phantoms as the benchmark does
+    params = learned.mlp_init(seed, *learned.intensity_stats(img))
+    for rnd in range(2):
+        for k in range(4):
+            ti, ta, _ = pct.synth_tissue(96, 96, 64, n, 3, False, 7000 + k)
+            params, trace = learned.mlp_train(torch.from_numpy(ti).to(dev),
+                                              torch.from_numpy(ta).to(dev), nh, sigma, beta,
+                                              params, steps=60, anchors=256, lr=3e-3,
+                                              seed=10 * rnd + k)
+            assert np.isfinite(trace).all()
+    out = torch.full(img.shape, -3.0, device=dev)
+    frac = learned.fuse_pruned_device(d_img, d_amri, d_lab, out, nh, sigma, beta, params)
+    o = out.cpu().numpy()
+    print(f"fallback fraction {frac}", flush=True)
+    assert 0.0 <= frac <= 1.0
+    assert np.array_equal(np.isnan(o), np.isnan(p))
+    r = (nh - 1) // 2 + 1
+    inner = (slice(r, nz - r), slice(r, nx - r), slice(r, ny - r))
+    dhu = np.abs(pct.fused_to_hu_raw(o[inner]) - pct.fused_to_hu_raw(p[inner]))
+    print(f"MAE {dhu.mean():.3f} HU, p99 {np.percentile(dhu, 99):.3f}, max {dhu.max():.3f}",
+          flush=True)
+    assert dhu.mean() <= 5.0
+    border = np.ones(img.shape, bool)
+    border[inner] = False
+    assert not o[border].any()

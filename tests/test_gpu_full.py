@@ -423,3 +423,51 @@ def test_cli_mat_file_round_trip(gpu, tmp_path):
     fused = pct.multi_patch_based_segmentation_gpu(img, amri, lab, nh, 2.5, 1.0)
     want = pct.postprocess(fused.copy(), nh)
     assert np.array_equal(np.transpose(got, (2, 0, 1)), want)
+
+
+def test_cli_cpu_surface_equals_gpu_surface(gpu, tmp_path):
+    # the CPU program's command line (data result EXEC_CHOICE NH TH sigma
+    # beta) and the GPU program's (data result EXEC_CHOICE NH sigma beta)
+    # give the same HU volume
+    nx, ny, nz, n, nh = 30, 28, 24, 2, 7
+    img, amri, act = pct.synth_tissue(nx, ny, nz, n, 2, False, 44)
+    paths = [str(tmp_path / f) for f in ("a.txt", "l.txt", "i.txt")]
+    pct.matrixio_write(paths[0], amri, n, nx, ny, nz)
+    pct.matrixio_write(paths[1], act, n, nx, ny, nz)
+    pct.matrixio_write(paths[2], img, 1, nx, ny, nz)
+    cfg = str(tmp_path / "data.cfg")
+    with open(cfg, "w") as f:
+        f.write("0\n" + "\n".join(paths) + "\n")
+    cli = os.path.join(os.path.dirname(_lib.LIB_PATH), "pct_cli")
+    outs = []
+    for k, argv in enumerate(([cli, "--cpu-surface", cfg, str(tmp_path / "cpu"), "1", str(nh),
+                               "4", "2.5", "1"],
+                              [cli, "--cpu-surface", cfg, str(tmp_path / "cpu2"), "2", str(nh),
+                               "2", "2.5", "1"],
+                              [cli, cfg, str(tmp_path / "gpu"), "2", str(nh), "2.5", "1"])):
+        r = subprocess.run(argv, capture_output=True, text=True, timeout=300)
+        print(r.stdout[-1500:], r.stderr[-1500:])
+        assert r.returncode == 0
+        if k == 1:   # SingleCoreTestTime: 10 samples of 10 calls
+            assert r.stdout.count("Iter ") == 10
+        hu, dims = pct.matrixio_read(argv[3 if k < 2 else 2] + ".mat")
+        assert dims == [nx, ny, nz, 1]
+        outs.append(hu)
+    assert np.array_equal(outs[0], outs[2]) and np.array_equal(outs[1], outs[2])
+
+
+def test_divisor_argument(gpu):
+    # div given directly (the reference wrapper's argument, cu:1635) equals
+    # the sigma / beta call bit for bit, and another div matches the oracle
+    img, amri, act = pct.synth_tissue(30, 34, 24, 2, 2, False, 45)
+    lab = act.copy()
+    pct.normalize_labels(lab)
+    a = pct.multi_patch_based_segmentation_gpu(img, amri, lab, 7, 2.5, 1.0)
+    b = pct.multi_patch_based_segmentation_gpu(img, amri, lab, 7, 0.0, 0.0,
+                                               div=pct.weight_divisor(2.5, 1.0))
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    div = 40.0
+    got = pct.multi_patch_based_segmentation_gpu(img, amri, lab, 7, div=div)
+    # sigma with 2 * 27 * sigma^2 = div, beta 1
+    ref = oracle.fuse(img, amri, lab, 7, float(np.sqrt(div / 54.0)), 1.0)
+    compare(got, ref)

@@ -277,3 +277,64 @@ def test_cli_fails_cleanly_without_device(tmp_path):
     r = subprocess.run([CLI, cfg, str(tmp_path / "out"), "2", "3"], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode != 0 and "gfx950" in r.stderr
+
+
+# ---- the CPU program's argument surface (cpp:1484-1565) ----------------------
+
+def _cli():
+    return os.path.join(os.path.dirname(_lib.LIB_PATH), "pct_cli")
+
+
+def test_cli_cpu_surface_parses_threads_sigma_beta(tmp_path):
+    # data result EXEC_CHOICE NH [TH] [sigma beta]: sigma and beta only with
+    # TH (cpp:1552-1558); the reference GPU surface would read TH as sigma
+    cfg = str(tmp_path / "rand.cfg")
+    with open(cfg, "w") as f:
+        f.write("1\n2\n12\n11\n10\n")
+    cases = [(["1", "11", "12", "2.5", "1"], ("MultiCore", "11", "12", "2.5", "1")),
+             (["0", "7", "4"], ("SingleCore", "7", "4", "1", "1")),
+             (["3", "9"], ("MultiCoreTestTime", "9", "1", "1", "1"))]
+    for argv, (choice, nh, th, sigma, beta) in cases:
+        r = subprocess.run([_cli(), "--cpu-surface", cfg, str(tmp_path / "res")] + argv,
+                           capture_output=True, text=True, timeout=60, cwd=str(tmp_path))
+        out = r.stdout
+        assert f"[3] Choice {choice}\n" in out, out
+        assert f"[4] NH {nh}\n[5] TH {th}\n[6] sigma {sigma}\n[7] beta {beta}\n" in out, out
+        # no GPU here: the run stops at the device, after reading the data
+        if pct.device_count() == 0:
+            assert r.returncode != 0 and "device 0" in r.stderr
+
+
+def test_cli_cpu_surface_dumps_random_inputs(tmp_path):
+    # random mode writes the generated inputs before the CT normalisation
+    # (cpp:974-978): atlas.txt and label.txt with num volumes, im.txt with one
+    cfg = str(tmp_path / "rand.cfg")
+    with open(cfg, "w") as f:
+        f.write("1\n2\n12\n11\n10\n")
+    subprocess.run([_cli(), "--cpu-surface", cfg, str(tmp_path / "res"), "1", "5", "2",
+                    "--seed", "5"], capture_output=True, text=True, timeout=60,
+                   cwd=str(tmp_path))
+    img, atlas, label = pct.synth_uniform(12, 11, 10, 2, 5)
+    for name, want, num in (("atlas.txt", atlas, 2), ("label.txt", label, 2), ("im.txt", img, 1)):
+        got, dims = pct.matrixio_read(str(tmp_path / name))
+        assert dims == [12, 11, 10, num]
+        # the text keeps 6 significant digits, as the reference's Store
+        np.testing.assert_allclose(got, want.reshape(-1), rtol=5e-6, atol=1e-6)
+    # the GPU surface does not write them (cu:1446-1449 has the writes commented out)
+    other = tmp_path / "gpu"
+    other.mkdir()
+    subprocess.run([_cli(), cfg, str(other / "res"), "2", "5"], capture_output=True, text=True,
+                   timeout=60, cwd=str(other))
+    assert not (other / "atlas.txt").exists()
+
+
+def test_problem_divisor_validation():
+    from ctypes import byref
+    L = _lib.lib()
+    ok = pct.problem((20, 20, 20), 1, 5, 0.0, 0.0, div=337.5)
+    assert L.pct_fuse_workspace_size(byref(ok), pct.CHOICE_SM) > 0
+    for bad in (pct.problem((20, 20, 20), 1, 5, 0.0, 0.0, div=0.0),
+                pct.problem((20, 20, 20), 1, 5, 2.5, 1.0, div=-1.0),
+                pct.problem((20, 20, 20), 1, 5, 2.5, 1.0, div=float("nan")),
+                pct.problem((20, 20, 20), 1, 5, 2.5, 1.0, div=float("inf"))):
+        assert L.pct_fuse_workspace_size(byref(bad), pct.CHOICE_SM) == 0
