@@ -188,7 +188,8 @@ def batched_leg(steps: int, warmup: int, seed: int):
 
 def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup, seed,
                 plain_ms, train_shape=(64, 96, 96), train_subjects=4, train_rounds=2,
-                train_steps=60, topk=32, samples=27, margin=0.0, max_slabs=3, shape_id=-1):
+                train_steps=60, topk=32, samples=27, margin=0.0, max_slabs=3, shape_id=-1,
+                max_err=None, params=None):
     """Learned-pruned fusion on the resident inputs: the embedding MLP is
     trained on the GPU on `train_subjects` other seeded phantoms, then the
     pruned fusion is timed and compared with the plain output."""
@@ -199,25 +200,30 @@ def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup,
     from pseudoctimaging_amd import learned
     dev = d_img.device
     n = d_amri.shape[0]
+    if max_err is None:
+        max_err = learned.DEFAULT_MAX_ERR
     tz, tx, ty = train_shape
     subjects = []
     for k in range(train_subjects):
         img, amri, _ = pct.synth_tissue(tx, ty, tz, n, 3, False, seed + 1000 + k)
         subjects.append((torch.from_numpy(img).to(dev), torch.from_numpy(amri).to(dev)))
     mu, inv_sd = learned.intensity_stats(d_img.cpu().numpy())
-    params = learned.mlp_init(seed, mu, inv_sd)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     losses = []
-    for rnd in range(train_rounds):
-        for k, (ti, ta) in enumerate(subjects):
-            params, trace = learned.mlp_train(ti, ta, nh, sigma, beta, params, steps=train_steps,
-                                              anchors=256, lr=3e-3, seed=seed + 10 * rnd + k)
-            losses.append([float(trace[0]), float(trace[-1])])
+    if params is None:
+        params = learned.mlp_init(seed, mu, inv_sd)
+        for rnd in range(train_rounds):
+            for k, (ti, ta) in enumerate(subjects):
+                params, trace = learned.mlp_train(ti, ta, nh, sigma, beta, params,
+                                                  steps=train_steps, anchors=256, lr=3e-3,
+                                                  seed=seed + 10 * rnd + k)
+                losses.append([float(trace[0]), float(trace[-1])])
     train_s = time.perf_counter() - t0
     del subjects
     out = torch.empty_like(plain_out)
-    kw = dict(topk=topk, samples=samples, margin=margin, max_slabs=max_slabs, shape_id=shape_id)
+    kw = dict(topk=topk, samples=samples, margin=margin, max_slabs=max_slabs, shape_id=shape_id,
+              max_err=max_err)
     frac = learned.fuse_pruned_device(d_img, d_amri, d_lab, out, nh, sigma, beta, params, **kw)
     for _ in range(warmup):
         learned.fuse_pruned_device(d_img, d_amri, d_lab, out, nh, sigma, beta, params,
@@ -240,7 +246,7 @@ def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup,
             float(np.percentile(dhu, 99)),
             "nan_mismatch": int(np.count_nonzero(np.isnan(a) != np.isnan(b))),
             "fallback_frac": frac, "topk": topk, "lattice_samples": samples, "margin": margin,
-            "max_slabs": max_slabs,
+            "max_err": max_err, "max_slabs": max_slabs, "params": params,
             "embedding": "30-64-64-16 MLP, BF16 MFMA inference, FP32 training on GPU",
             "train": {"subjects": train_subjects, "shape_zxy": list(train_shape),
                       "rounds": train_rounds, "steps_per_subject": train_steps,
@@ -485,6 +491,7 @@ def main() -> int:
         log("learned leg: training the embedding on 4 other phantoms, then pruned fusion")
         learned_res = learned_leg(d_img, d_amri, d_lab, d_out, nh, sigma, beta,
                                   max(2, args.steps // 2), 1, args.seed, ms_per_step)
+        learned_res.pop("params")
         log(f"learned: {learned_res['ms_per_step']:.2f} ms/step, speedup "
             f"{learned_res['speedup_vs_plain']:.2f}x, MAE {learned_res['mae_hu_vs_plain']:.3f} HU, "
             f"fallback {learned_res['fallback_frac']:.3f}")

@@ -285,6 +285,14 @@ typedef struct pct_pruned_opts {
                         (bounds the LDS staging per tile); 0 = any */
     int embed_all;   /* 1: embed every atlas voxel instead of only the probe
                         windows the selection reads (same result; checking) */
+    float max_err;   /* > 0: tiles in which some voxel's estimated error
+                        exceeds max_err (label units; 1 = 16.06 HU) take the
+                        full search.  The tail term replaces the weighted CT
+                        mean of the candidates it stands for by their plain
+                        mean, which misses by cov(w, L) / mean(w) over them;
+                        the estimate is that ratio over the lattice samples'
+                        exact weights, times the tail's share of the weight
+                        sum.  <= 0: no estimate */
 } pct_pruned_opts;
 
 /* Learned-pruned fusion.  Per tile of target voxels and per atlas, the
@@ -292,7 +300,9 @@ typedef struct pct_pruned_opts {
  * compact probe cluster at the tile centre) are weighted exactly in FP32 for
  * every voxel of the tile; the remaining candidates enter through their CT
  * window sum times the mean exact weight of `samples` lattice offsets.  Tiles
- * with a low selection margin run the full exact search.  Output as
+ * with a low selection margin, or (max_err > 0) with a voxel whose estimated
+ * error exceeds max_err, run the full exact search and give exactly the
+ * plain kernel's sums for that tile and atlas.  Output as
  * pct_fuse_dev.  If fallback_frac is not NULL the call synchronises and
  * stores the fraction of (tile, atlas) items that fell back.  Planes of
  * nx*ny >= 2^31 voxels return PCT_ERR_TOO_LARGE.  Part of the work runs on

@@ -76,7 +76,7 @@ class VolumeSet(Structure):
 class PrunedOpts(Structure):
     """pct_pruned_opts"""
     _fields_ = [("topk", c_int), ("samples", c_int), ("margin", c_float), ("shape_id", c_int),
-                ("max_slabs", c_int), ("embed_all", c_int)]
+                ("max_slabs", c_int), ("embed_all", c_int), ("max_err", c_float)]
 
 
 class TrainOpts(Structure):

@@ -1273,10 +1273,11 @@ __global__ __launch_bounds__(kSelThreads) void select_kernel(SelectArgs a) {
 struct PrunedArgs {
     TiledArgs t;                    // problem, tiles (group = 1), workspace partials
     const int* sel;                 // [items][topk]
-    const unsigned char* flag;      // [items]
+    unsigned char* flag;            // [items]: 1 flagged by the selection, 2 by the error check
     const float* boxL;              // [n][N] window sums of the CT
     const int* merged;              // [items][kMergedStride] offset lists (merge_offsets_kernel)
     int ncand;                      // NH^3
+    float max_err;                  // > 0: error bound (label units) above which an item is flagged
 };
 
 // Per (tile, atlas) item, the offsets the pruned kernel evaluates, in its
@@ -1337,13 +1338,23 @@ __global__ __launch_bounds__(kMergeWaves * 64) void merge_offsets_kernel(MergeAr
     }
 }
 
+// Sums over the lattice samples of the lane's first output (z0, x = xb),
+// for the error estimate of the tail term: w L and L (the sum of w is Ws).
+// One output in eight of every tile is estimated, spread over all its rows
+// and columns: enough to flag the tile, for two registers.
+struct SampleMoments {
+    float wl, l1;
+};
+
 // As tiled_step, also summing the CT of the evaluated candidates (Le) and
-// the weights of the lattice samples (Ws).
-template <class C, int BX, int BZ, bool SAMPLE>
+// the weights of the lattice samples (Ws); with ERR also the moments of the
+// samples.
+template <class C, int BX, int BZ, bool SAMPLE, bool ERR>
 __device__ __forceinline__ void pruned_step(const float* __restrict__ lds, int base,
                                             const float (&tv)[BZ + 2][BX + 2],
                                             float (&P)[BZ][BX], float (&S)[BZ][BX],
-                                            float (&Le)[BZ][BX], float (&Ws)[BZ][BX]) {
+                                            float (&Le)[BZ][BX], float (&Ws)[BZ][BX],
+                                            SampleMoments& m) {
     constexpr int NBA = (BZ + 2 + C::PPB - 1) / C::PPB;
     constexpr int NBL = (BZ + C::PPB - 1) / C::PPB;
     int ba[NBA];
@@ -1392,6 +1403,10 @@ __device__ __forceinline__ void pruned_step(const float* __restrict__ lds, int b
             if (SAMPLE) {
                 Ws[j][i] += w0;
                 Ws[j + 1][i] += w1;
+            }
+            if (SAMPLE && ERR && j == 0 && i == 0) {
+                m.wl = fmaf(w0, l0, m.wl);
+                m.l1 += l0;
             }
         }
     }
@@ -1444,10 +1459,12 @@ __global__ __launch_bounds__(NW * 64) void pruned_kernel(PrunedArgs pa) {
     float tv[BZ + 2][BX + 2];
     load_target_block<BX, BZ>(args.img, args.scale, z0, xb, y, nx, ny, nz, plane, tv);
     float P[BZ][BX], S[BZ][BX], Le[BZ][BX], Ws[BZ][BX];
+    SampleMoments mom{0.f, 0.f};
 #pragma unroll
     for (int j = 0; j < BZ; ++j)
 #pragma unroll
         for (int i = 0; i < BX; ++i) P[j][i] = S[j][i] = Le[j][i] = Ws[j][i] = 0.f;
+    const bool err = pa.max_err > 0.f;   // uniform over the grid
 
     const int lane_base = sg * BX * C::CW + ly;
     const int xs = x0 - 1 - HW;
@@ -1463,12 +1480,20 @@ __global__ __launch_bounds__(NW * 64) void pruned_kernel(PrunedArgs pa) {
         __syncthreads();
         stage_slab_direct<C, NW>(lds, A, L, args.scale, z0 + dz, nz, plane, wv, dp);
         for (; k < m && odz[k] == dz && !is_sample[k]; ++k)
-            pruned_step<C, BX, BZ, false>(lds, lane_base + odel[k], tv, P, S, Le, Ws);
-        for (; k < m && odz[k] == dz; ++k)
-            pruned_step<C, BX, BZ, true>(lds, lane_base + odel[k], tv, P, S, Le, Ws);
+            pruned_step<C, BX, BZ, false, false>(lds, lane_base + odel[k], tv, P, S, Le, Ws, mom);
+        if (err) {
+            for (; k < m && odz[k] == dz; ++k)
+                pruned_step<C, BX, BZ, true, true>(lds, lane_base + odel[k], tv, P, S, Le, Ws,
+                                                   mom);
+        } else {
+            for (; k < m && odz[k] == dz; ++k)
+                pruned_step<C, BX, BZ, true, false>(lds, lane_base + odel[k], tv, P, S, Le, Ws,
+                                                    mom);
+        }
     }
 
     // tail: the candidates not evaluated, at the mean sample weight
+    int over = 0;
     if (ly >= 1 && ly <= W - 2 && y < ny - r) {
         const int ns = n_samp_eff;
         const float n_rest = (float)(pa.ncand - m);
@@ -1483,11 +1508,26 @@ __global__ __launch_bounds__(NW * 64) void pruned_kernel(PrunedArgs pa) {
                 if (z < nz - r && x < nx - r) {
                     const long long idx = z * plane + (long long)x * ny + y;
                     const float wbar = ns > 0 ? Ws[j][i] / (float)ns : 0.f;
+                    const float S_tail = wbar * n_rest;
+                    const float S_all = fmaf(wbar, n_rest, S[j][i]);
                     wsP[idx] = fmaf(wbar, bL[idx] - Le[j][i], P[j][i]);
-                    wsS[idx] = fmaf(wbar, n_rest, S[j][i]);
+                    wsS[idx] = S_all;
+                    if (err && ns > 1 && j == 0 && i == 0) {
+                        // the tail stands for its candidates' weighted CT mean
+                        // by their plain mean: it misses by cov(w, L) / mean(w)
+                        // over them, estimated from the samples, times the
+                        // tail's share of the weight
+                        const float share = S_all > 0.f ? S_tail / S_all : 1.f;
+                        const float cov_ratio =
+                            Ws[0][0] > 0.f ? fabsf(fmaf(-wbar, mom.l1, mom.wl)) / Ws[0][0] : 0.f;
+                        const float e = share * cov_ratio;
+                        // an exhausted weight sum cannot be estimated at all
+                        over |= !(e <= pa.max_err) || !(S_all > 0.f);
+                    }
                 }
             }
     }
+    if (err && __syncthreads_or(over) && threadIdx.x == 0) pa.flag[item] = 2;
 }
 
 using PrunedFn = void (*)(PrunedArgs);
@@ -1768,6 +1808,7 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
     pa.flag = sa.flag;
     pa.boxL = boxL;
     pa.ncand = ncand;
+    pa.max_err = opts->max_err > 0.f ? opts->max_err : 0.f;
     // the evaluated offsets of every item, merged once (one wave per item)
     {
         MergeArgs ma;
@@ -1797,9 +1838,10 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
             return PCT_ERR_HIP;
     }
 
-    // 5. full exact search for the flagged items (with margin 0 the
-    // selection flags none, and the launch is skipped)
-    if (opts->margin != 0.f) {
+    // 5. full exact search for the flagged items, by the selection's margin
+    // or the pruned kernel's error estimate (with neither, nothing is
+    // flagged and the launch is skipped)
+    if (opts->margin != 0.f || pa.max_err > 0.f) {
         TiledArgs fa = pa.t;
         fa.only = sa.flag;
         rc = launch_tiled_items(fa, shape_id, hw, s);
@@ -1817,7 +1859,7 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
             return PCT_ERR_HIP;
         if (hipStreamSynchronize(s) != hipSuccess) return PCT_ERR_HIP;
         long long cnt = 0;
-        for (unsigned char c : fl) cnt += c;
+        for (unsigned char c : fl) cnt += c != 0;
         *fallback_frac = items > 0 ? (double)cnt / (double)items : 0.0;
     }
     return PCT_OK;

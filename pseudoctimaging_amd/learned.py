@@ -79,23 +79,33 @@ def bf16_words_to_float(words) -> np.ndarray:
     return w.view(np.float32)
 
 
+# Error bar of the pruned fusion's fallback, in label units (16 HU): at
+# 222x222x112, n=10, NH=11 it flags no tile at sigma 2.5 (1.90 HU MAE against
+# the exact fusion, 7.9x faster) and every tile at sigma 1 and 0.25, where the
+# pruned sums would miss by 11.6 and 124 HU MAE.
+DEFAULT_MAX_ERR = 1.0
+
+
 def fuse_pruned_device(img, atlas, atlas_label, out, nh: int, sigma: float, beta: float,
                        params: np.ndarray, topk: int = 32, samples: int = 27,
                        margin: float = 0.0, shape_id: int = -1, max_slabs: int = 3,
-                       embed_all: bool = False, stream=None, report_fallback: bool = True):
+                       embed_all: bool = False, stream=None, report_fallback: bool = True,
+                       max_err: float = DEFAULT_MAX_ERR):
     """Learned-pruned fusion on device tensors.  Returns the fraction of
     (tile, atlas) items that took the full search (synchronises) when
     report_fallback, else None (asynchronous).  max_slabs bounds the dz
     planes the kept offsets come from (0: any).  The default 3 keeps them on
     the lattice samples' planes: at 222x222x112, n=10, NH=11 it measured
-    faster than 4 (3.49 vs 3.80 ms) at the same error (1.90 HU MAE)."""
+    faster than 4 (3.49 vs 3.80 ms) at the same error (1.90 HU MAE).
+    max_err (label units) is the bar of the per-voxel error estimate above
+    which a tile takes the full search; 0 switches the estimate off."""
     params = _f32(params, "params")
     check_volume_set(img, atlas, atlas_label, out, stream)
     s = _stream(img, stream)
     pb = problem(img.shape, atlas.shape[0], nh, sigma, beta)
     opts = PrunedOpts(topk=int(topk), samples=int(samples), margin=float(margin),
                       shape_id=int(shape_id), max_slabs=int(max_slabs),
-                      embed_all=int(bool(embed_all)))
+                      embed_all=int(bool(embed_all)), max_err=float(max_err))
     frac = ctypes.c_double(-1.0)
     check(lib().pct_fuse_pruned_dev(byref(pb), fptr(params), byref(opts),
                                     _dev_ptr(img, "img"), _dev_ptr(atlas, "atlas"),
