@@ -256,6 +256,47 @@ def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup,
                      "+ fallback + finalize, per call"}
 
 
+def nan_fill_leg(seed, reps=3):
+    """The post chain on a NaN-heavy full-size volume: the reference's random
+    mode (uniform 10 U[0, 1) data, cpp:961-973) at 222x222x112, n=2, NH=3,
+    sigma=0.33, where most voxels have no weight above DBL_EPSILON.  The
+    device chain (pct_postprocess_dev: NaN list, wavefront-ordered fill,
+    round, HU) against the host chain (pct_postprocess) on the same fused
+    volume; both must agree bit for bit."""
+    import numpy as np
+    import torch
+
+    import pseudoctimaging_amd as pct
+    nx, ny, nz, n, nh, sigma = 222, 222, 112, 2, 3, 0.33
+    img, amri, lab = pct.synth_uniform(nx, ny, nz, n, seed)
+    pct.normalize_labels(lab)
+    dev = torch.device("cuda:0")
+    d = [torch.from_numpy(x).to(dev) for x in (img, amri, lab)]
+    fused_d = torch.empty(img.shape, device=dev)
+    pct.fuse_device(*d, fused_d, nh, sigma, 1.0)
+    torch.cuda.synchronize()
+    fused = fused_d.cpu().numpy()
+    n_nan = int(np.isnan(fused).sum())
+    t_dev = []
+    for _ in range(reps):
+        t = fused_d.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pct.postprocess_device(t, nh)
+        torch.cuda.synchronize()
+        t_dev.append(time.perf_counter() - t0)
+    got = t.cpu().numpy()
+    host = fused.copy()
+    t0 = time.perf_counter()
+    pct.postprocess(host, nh)
+    t_host = time.perf_counter() - t0
+    return {"shape_zxy": [nz, nx, ny], "n": n, "nh": nh, "sigma": sigma,
+            "nan_voxels": n_nan, "nan_fraction": n_nan / fused.size,
+            "device_chain_ms": float(np.median(t_dev)) * 1e3, "host_chain_ms": t_host * 1e3,
+            "same_output": bool(np.array_equal(got.view(np.uint32), host.view(np.uint32))),
+            "data": "random mode: uniform 10 U[0,1) target, atlas MRI and CT (seeded)"}
+
+
 def end_to_end_leg(img, amri, lab, nh, sigma, beta, reps):
     """The region the reference's GPU program times around its SM call
     (src/GPU/PatchBasedPseudoCT.cu:2152-2188): host buffers in, three
@@ -326,6 +367,8 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-pruned leg")
     ap.add_argument("--no-end-to-end", action="store_true",
                     help="skip the host-buffer end-to-end leg")
+    ap.add_argument("--no-nan-fill", action="store_true",
+                    help="skip the NaN-heavy post-chain leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized float32 buffer three times (a plain copy "
                          "kernel of known size, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -486,6 +529,14 @@ def main() -> int:
             f"post {e2e['post_host_ms']:.1f} ms); pct_pseudo_ct_host "
             f"{e2e['pseudo_ct_host_ms']:.1f} ms, same output {e2e['pseudo_ct_host_same_output']}")
 
+    nan_fill = None
+    if not args.no_nan_fill:
+        log("NaN-fill leg: random-mode volume at sigma 0.33, device vs host post chain")
+        nan_fill = nan_fill_leg(args.seed)
+        log(f"NaN fill: {nan_fill['nan_voxels']} NaN voxels; device chain "
+            f"{nan_fill['device_chain_ms']:.1f} ms, host chain {nan_fill['host_chain_ms']:.1f} ms, "
+            f"same output {nan_fill['same_output']}")
+
     learned_res = None
     if not args.no_learned:
         log("learned leg: training the embedding on 4 other phantoms, then pruned fusion")
@@ -555,6 +606,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "parity_sample": parity,
         "end_to_end": e2e,
+        "nan_fill": nan_fill,
         "learned_pruned": learned_res,
         "batched": batched,
     }

@@ -5,13 +5,23 @@
 //
 // RegularizationStep sweeps the interior in z, x, y order and overwrites each
 // NaN voxel in place, so a voxel sees the values filled earlier in the same
-// sweep.  Only NaN voxels ever change, so the sweep is reproduced exactly by
-// (1) listing the interior NaN voxels in scan order (per-chunk counts, a
-// scan, an ordered compaction per chunk) and (2) filling them in list order with one wave (the 27
-// neighbours on 27 lanes, the median by stable ranks across lanes); voxels
-// whose median is still NaN form the next sweep's list.  Sweeps stop when no
-// NaN is left or a sweep fills nothing.  Rounding and the rescale are
-// elementwise.
+// sweep.  Only NaN voxels ever change, and a NaN voxel's median reads its
+// neighbours: those before it in scan order as this sweep left them, those
+// after it as the previous sweep did (a NaN one after it is still NaN when
+// it is reached).  With the wavefront index t = 4z + 2x + y every neighbour
+// before a voxel in scan order has a smaller t and every neighbour after it
+// a larger one (the steepest neighbours differ by (0, 0, +-1), (0, +-1, -+1)
+// and (+-1, -+1, -+1): t steps of 1, 1 and 1).  Filling the sweep's NaN
+// voxels in increasing t, all voxels of one t at once, therefore gives the
+// sequential sweep's values bit for bit.
+//
+// (1) The interior NaN voxels are listed (per-chunk counts, a scan, a
+// compaction per chunk); (2) one workgroup runs the sweeps: a counting sort
+// of the list by t, then the levels in increasing t with a barrier after
+// each non-empty one, one voxel per thread (the median by stable ranks, the
+// std::stable_sort of cpp:1035, signed zeros included).  Voxels whose median
+// is still NaN form the next sweep's list; sweeps stop when no NaN is left
+// or a sweep fills nothing.  Rounding and the rescale are elementwise.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -122,73 +132,124 @@ __global__ __launch_bounds__(kListThreads) void nan_write_kernel(const float* __
     }
 }
 
-// Value of the valid lane whose stable rank is `pos` (all lanes receive it).
-__device__ __forceinline__ float value_at_rank(float v, bool ok, int rank, int pos) {
-    const unsigned long long sel = __ballot(ok && rank == pos);
-    const int src = __ffsll((long long)sel) - 1;
-    return __shfl(v, src);
+constexpr int kFillThreads = 1024;
+
+// Median of the non-NaN values among the 27 neighbourhood values of `id`
+// (cpp:1020-1042): the element(s) of rank cnt/2 (and cnt/2 - 1) in a stable
+// ascending order, the even-count mean in double; NaN without a value.
+__device__ __forceinline__ float neighbourhood_median(const float* arr, int id, int pl, int ny) {
+    float v[27];
+#pragma unroll
+    for (int k = 0; k < 27; ++k)
+        v[k] = __hip_atomic_load(arr + id + (k / 9 - 1) * pl + ((k / 3) % 3 - 1) * ny + (k % 3 - 1),
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 27; ++k) cnt += isnan(v[k]) ? 0 : 1;
+    if (cnt == 0) return __builtin_nanf("");
+    const int hi = cnt / 2, lo_rank = cnt / 2 - 1;
+    float a = 0.f, b = 0.f;
+#pragma unroll
+    for (int i = 0; i < 27; ++i) {
+        if (isnan(v[i])) continue;
+        int rank = 0;
+#pragma unroll
+        for (int j = 0; j < 27; ++j)
+            rank += (!isnan(v[j]) && (v[j] < v[i] || (v[j] == v[i] && j < i))) ? 1 : 0;
+        if (rank == hi) b = v[i];
+        if (rank == lo_rank) a = v[i];
+    }
+    if (cnt % 2 == 0) return (float)(((double)a + (double)b) / 2);
+    return b;
 }
 
-// The sweeps of RegularizationStep over the listed voxels: one wave.
-// Neighbour values are read with agent-scope atomic loads and each fill is
-// stored the same way and fenced, so a later voxel of the same sweep sees it.
-__global__ __launch_bounds__(64) void nan_fill_kernel(float* __restrict__ arr, int* list_a,
-                                                      int* list_b, const int* count0, int nx,
-                                                      int ny, int* sweeps_out) {
-    const int lane = threadIdx.x;
+// The sweeps of RegularizationStep over the listed voxels, in one workgroup.
+// hist has t_count + 1 ints; sorted and the lists hold `interior` ints.
+// Values and list entries go through agent-scope atomics, and every level
+// ends with a fence and a barrier, so a level sees the ones before it.
+__global__ __launch_bounds__(kFillThreads) void nan_fill_levels_kernel(
+    float* __restrict__ arr, int* list_a, int* list_b, int* sorted, int* hist,
+    const int* count0, int nx, int ny, int t_count, int* sweeps_out) {
+    __shared__ int s_n, s_next, s_filled;
+    __shared__ int s_end[kFillThreads];   // level ends of one chunk of levels
     const int pl = nx * ny;
-    const int dr = lane / 9 - 1, ds = (lane / 3) % 3 - 1, dt = lane % 3 - 1;
-    const int noff = lane < 27 ? dr * pl + ds * ny + dt : 0;
     int* cur = list_a;
     int* nxt = list_b;
-    int n = __hip_atomic_load(count0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) s_n = __hip_atomic_load(count0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    int n = s_n;
     int sweeps = 0;
     while (n > 0) {
         ++sweeps;
-        int m = 0, filled = 0;
-        for (int i = 0; i < n; ++i) {
-            const int id = __hip_atomic_load(cur + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            float v = __builtin_nanf("");
-            if (lane < 27)
-                v = __hip_atomic_load(arr + id + noff, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const bool ok = lane < 27 && !isnan(v);
-            const unsigned long long bal = __ballot(ok);
-            const int cnt = __popcll(bal);
-            float res = __builtin_nanf("");
-            if (cnt > 0) {
-                // rank in a stable ascending sort of the valid values (Median, cpp:1035)
-                int rank = 0;
-                for (int j = 0; j < 27; ++j) {
-                    const float u = __shfl(v, j);
-                    const bool okj = (bal >> j) & 1ull;
-                    rank += (okj && (u < v || (u == v && j < lane))) ? 1 : 0;
-                }
-                if (cnt % 2 == 0) {
-                    const double a = value_at_rank(v, ok, rank, cnt / 2 - 1);
-                    const double b = value_at_rank(v, ok, rank, cnt / 2);
-                    res = (float)((a + b) / 2);
-                } else {
-                    res = value_at_rank(v, ok, rank, cnt / 2);
-                }
-            }
-            if (lane == 0) {
-                __hip_atomic_store(arr + id, res, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (isnan(res))
-                    __hip_atomic_store(nxt + m, id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            __threadfence();
-            if (isnan(res))
-                ++m;
-            else
-                ++filled;
+        // counting sort of the list by t = 4 z + 2 x + y
+        for (int t = threadIdx.x; t <= t_count; t += kFillThreads) hist[t] = 0;
+        __threadfence();
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += kFillThreads) {
+            const int id = cur[i];
+            const int z = id / pl, x = (id - z * pl) / ny, y = id - z * pl - x * ny;
+            atomicAdd(hist + 4 * z + 2 * x + y, 1);
         }
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // exclusive prefix of the counts; hist[t] becomes the level start
+            int run = 0;
+            for (int t = 0; t <= t_count; ++t) {
+                const int c = hist[t];
+                hist[t] = run;
+                run += c;
+            }
+            s_next = 0;
+            s_filled = 0;
+        }
+        __threadfence();
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += kFillThreads) {
+            const int id = cur[i];
+            const int z = id / pl, x = (id - z * pl) / ny, y = id - z * pl - x * ny;
+            sorted[atomicAdd(hist + 4 * z + 2 * x + y, 1)] = id;
+        }
+        __threadfence();
+        __syncthreads();
+        // hist[t] is now the end of level t; levels in increasing t, their
+        // ends read into LDS a chunk at a time
+        int begin = 0;
+        for (int t0 = 0; t0 < t_count; t0 += kFillThreads) {
+            if (t0 + (int)threadIdx.x < t_count)
+                s_end[threadIdx.x] = __hip_atomic_load(hist + t0 + threadIdx.x, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+            __syncthreads();
+            const int tn = min(kFillThreads, t_count - t0);
+            for (int k = 0; k < tn; ++k) {
+                const int end = s_end[k];
+                if (end == begin) continue;   // uniform: an empty level
+                for (int i = begin + (int)threadIdx.x; i < end; i += kFillThreads) {
+                    const int id = __hip_atomic_load(sorted + i, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                    const float res = neighbourhood_median(arr, id, pl, ny);
+                    __hip_atomic_store(arr + id, res, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (isnan(res))
+                        __hip_atomic_store(nxt + atomicAdd(&s_next, 1), id, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    else
+                        atomicAdd(&s_filled, 1);
+                }
+                __threadfence();
+                __syncthreads();
+                begin = end;
+            }
+            __syncthreads();   // s_end is rewritten for the next chunk
+        }
+        const int m = s_next, filled = s_filled;
+        __syncthreads();   // every thread read the counters
         if (m == 0 || filled == 0) break;
-        int* t = cur;
+        int* tmp = cur;
         cur = nxt;
-        nxt = t;
+        nxt = tmp;
         n = m;
     }
-    if (lane == 0 && sweeps_out) *sweeps_out = sweeps;
+    if (threadIdx.x == 0 && sweeps_out) *sweeps_out = sweeps;
 }
 
 // RoundMatrix (cpp:1221-1227) then round(4095 * (v / 255)) - 1024 in float
@@ -215,13 +276,19 @@ static long long fill_interior(int nh, int nx, int ny, int nz) {
     return (iz > 0 && ix > 0 && iy > 0) ? iz * ix * iy : 0;
 }
 
-// Workspace of the NaN fill: a counter, two NaN lists and per-chunk counts
-// and offsets.
-static size_t fill_ws_bytes(long long interior) {
+// Wavefront levels of a volume: t = 4 z + 2 x + y ranges over [0, t_count).
+static long long fill_levels(int nx, int ny, int nz) {
+    return 4LL * (nz - 1) + 2LL * (nx - 1) + (ny - 1) + 1;
+}
+
+// Workspace of the NaN fill: a counter, three voxel lists (two sweeps' and
+// the sorted one), per-chunk counts and offsets, and the level histogram.
+static size_t fill_ws_bytes(long long interior, long long levels) {
     const size_t list_bytes = ((size_t)interior * 4 + 255) & ~(size_t)255;
     const long long nb = (interior + kListChunk - 1) / kListChunk;
     const size_t block_bytes = ((size_t)nb * 4 + 255) & ~(size_t)255;
-    return 256 + 2 * list_bytes + 2 * block_bytes;
+    const size_t hist_bytes = ((size_t)(levels + 1) * 4 + 255) & ~(size_t)255;
+    return 256 + 3 * list_bytes + 2 * block_bytes + hist_bytes;
 }
 
 static int enqueue_fill(int nh, int nx, int ny, int nz, float* d_arr, void* ws, hipStream_t s) {
@@ -236,16 +303,19 @@ static int enqueue_fill(int nh, int nx, int ny, int nz, float* d_arr, void* ws, 
     int* count = reinterpret_cast<int*>(w);
     int* la = reinterpret_cast<int*>(w + 256);
     int* lb = reinterpret_cast<int*>(w + 256 + list_bytes);
-    int* bcount = reinterpret_cast<int*>(w + 256 + 2 * list_bytes);
-    int* boff = reinterpret_cast<int*>(w + 256 + 2 * list_bytes + block_bytes);
+    int* srt = reinterpret_cast<int*>(w + 256 + 2 * list_bytes);
+    int* bcount = reinterpret_cast<int*>(w + 256 + 3 * list_bytes);
+    int* boff = reinterpret_cast<int*>(w + 256 + 3 * list_bytes + block_bytes);
+    int* hist = reinterpret_cast<int*>(w + 256 + 3 * list_bytes + 2 * block_bytes);
+    const long long levels = fill_levels(nx, ny, nz);
     const int ixi = (int)ix, iyi = (int)iy;
     hipLaunchKernelGGL(nan_count_kernel, dim3((unsigned)nb), dim3(kListThreads), 0, s, d_arr, nx,
                        ny, lo, ixi, iyi, interior, bcount);
     hipLaunchKernelGGL(nan_scan_kernel, dim3(1), dim3(1024), 0, s, bcount, (int)nb, boff, count);
     hipLaunchKernelGGL(nan_write_kernel, dim3((unsigned)nb), dim3(kListThreads), 0, s, d_arr, nx,
                        ny, lo, ixi, iyi, interior, boff, la);
-    hipLaunchKernelGGL(nan_fill_kernel, dim3(1), dim3(64), 0, s, d_arr, la, lb, count, nx, ny,
-                       (int*)nullptr);
+    hipLaunchKernelGGL(nan_fill_levels_kernel, dim3(1), dim3(kFillThreads), 0, s, d_arr, la, lb,
+                       srt, hist, count, nx, ny, (int)levels, (int*)nullptr);
     return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;
 }
 
@@ -257,7 +327,7 @@ static int enqueue_fill_lib(int nh, int nx, int ny, int nz, float* d_arr, hipStr
     const long long interior = fill_interior(nh, nx, ny, nz);
     if (interior == 0) return PCT_OK;
     void* ws = nullptr;
-    rc = ensure_workspace(dev, fill_ws_bytes(interior), &ws);
+    rc = ensure_workspace(dev, fill_ws_bytes(interior, fill_levels(nx, ny, nz)), &ws);
     if (rc) return rc;
     return enqueue_fill(nh, nx, ny, nz, d_arr, ws, s);
 }
@@ -270,7 +340,7 @@ static int enqueue_round_hu(int nx, int ny, int nz, float* d_arr, hipStream_t s)
 }
 
 size_t postprocess_ws_bytes(int nh, int nx, int ny, int nz) {
-    return fill_ws_bytes(fill_interior(nh, nx, ny, nz));
+    return fill_ws_bytes(fill_interior(nh, nx, ny, nz), fill_levels(nx, ny, nz));
 }
 
 int postprocess_dev_ws(int nh, int nx, int ny, int nz, float* d_arr, void* ws, hipStream_t s) {

@@ -108,3 +108,37 @@ def test_pseudo_ct_host_equals_fusion_then_host_chain(gpu, name):
     one = pct.pseudo_ct_hu(img, amri, lab, nh, sigma, beta)
     two = pct.postprocess(fused, nh)
     assert _same(one, two)
+
+
+@pytest.mark.parametrize("shape,density", [((40, 46, 52), 0.8), ((30, 120, 40), 0.97),
+                                           ((64, 64, 64), 0.3)])
+def test_dense_nan_fill_matches_host(gpu, shape, density):
+    # NaN-dense volumes: long chains of voxels that each wait on a voxel
+    # filled earlier in the same sweep (the wavefront levels run in order),
+    # several sweeps, ties and signed zeros among the values
+    rng = np.random.default_rng(int(density * 100) + shape[0])
+    a = (rng.integers(0, 9, shape).astype(np.float32) - 4.0) * 0.25
+    a[rng.random(shape) < 0.1] = -0.0
+    a[rng.random(shape) < density] = np.nan
+    print(f"case dense fill {shape} density {density}: {int(np.isnan(a).sum())} NaN", flush=True)
+    for nh in (1, 5):
+        got = _gpu_post(a.copy(), nh, full=False)
+        host = a.copy()
+        pct.regularization_step(host, nh)
+        assert _same(got, host), nh
+        full = _gpu_post(a.copy(), nh)
+        assert _same(full, pct.postprocess(a.copy(), nh)), nh
+
+
+def test_random_mode_fusion_fill_matches_host(gpu):
+    # the reference's random mode (uniform 10 U[0, 1) data) at a small sigma:
+    # most interior voxels come out NaN, and the device chain reproduces the
+    # host chain bit for bit
+    nx, ny, nz, n, nh = 60, 64, 40, 2, 3
+    img, amri, lab = pct.synth_uniform(nx, ny, nz, n, 17)
+    pct.normalize_labels(lab)
+    fused = pct.multi_patch_based_segmentation_gpu(img, amri, lab, nh, 0.33, 1.0)
+    print(f"case random mode: {int(np.isnan(fused).sum())} NaN of {fused.size}", flush=True)
+    assert np.isnan(fused).mean() > 0.3
+    got = _gpu_post(fused.copy(), nh)
+    assert _same(got, pct.postprocess(fused.copy(), nh))
