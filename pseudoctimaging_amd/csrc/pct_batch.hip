@@ -86,8 +86,13 @@ __global__ __launch_bounds__(NW * 64) void fuse_batched(BatchArgs args) {
 }
 
 // fuse_batched with the cross-term items (TiledArgs::aa filled per subject).
+// The item loop costs the compiler some 18 VGPRs over the one-item kernel;
+// past 128 a SIMD holds three waves instead of four, which the LDS of the
+// narrow shapes (37.6 KB at W 32, BX 2, BZ 2, HW 5) would allow, and the
+// class runs a quarter slower.  Four waves per SIMD caps it at 128.
 template <int W, int BX, int BZ, int NW, int HW>
-__global__ __launch_bounds__(NW * 64) void fuse_batched_xt(BatchArgs args) {
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4)))
+void fuse_batched_xt(BatchArgs args) {
     extern __shared__ __attribute__((aligned(16))) float lds[];
     __shared__ long long claimed;
     const int home = blockIdx.x & (kChunks - 1);
@@ -153,6 +158,7 @@ static const std::vector<BatchKernel>& batch_kernels() {
         make_batch_kernel<32, 2, 4, 8>(13),
         make_batch_kernel_xt<32, 2, 2, 4>(16),
         make_batch_kernel_xt<32, 2, 2, 8>(17),
+        make_batch_kernel_xt<64, 2, 2, 4>(19),
     };
     return k;
 }

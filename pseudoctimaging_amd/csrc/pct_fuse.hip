@@ -113,49 +113,93 @@ __global__ __launch_bounds__(256) void fuse_direct(const float* __restrict__ img
 }
 
 // AA of every voxel of atlases [0, n_atlas): the 3x3x3 sum of (scale * A)^2,
-// zero outside the volume.  A workgroup owns 64 y x 4 x columns and sweeps z:
-// each plane's (4+2) x (64+2) tile goes through LDS for the in-plane 3x3
-// sums, which enter a running sum over three planes.
-constexpr int kSqBoxY = 64, kSqBoxX = 4;
-__global__ __launch_bounds__(kSqBoxY * kSqBoxX) void atlas_sq_box(const float* __restrict__ atlas,
-                                                                  long long N, int nx, int ny,
-                                                                  int nz, int n_atlas,
-                                                                  float scale,
-                                                                  float* __restrict__ aa) {
-    __shared__ float tile[kSqBoxX + 2][kSqBoxY + 2];
+// zero outside the volume.  A workgroup owns 64 y x 4 x columns of one chunk
+// of zc planes and sweeps it in z: the in-plane 3x3 sums of each plane's
+// (4+2) x (64+2) tile, staged through LDS, enter a running sum over three
+// planes.  Four planes are loaded per step (eight loads in flight per lane
+// instead of one, issued before the previous step's sums), and chunking z (two halo planes per chunk) gives the grid
+// enough workgroups to hide HBM latency on a single subject.  Every output
+// is ((s(z-1) + s(z)) + s(z+1)) with each in-plane sum in (dx, dy) order,
+// whatever the chunking.
+constexpr int kSqBoxY = 64, kSqBoxX = 4, kSqStep = 4;
+constexpr int kSqThreads = kSqBoxY * kSqBoxX;
+constexpr int kSqTile = (kSqBoxX + 2) * (kSqBoxY + 2);
+constexpr int kSqLoads = (kSqTile + kSqThreads - 1) / kSqThreads;
+constexpr int kSqMinChunk = 8;
+__global__ __launch_bounds__(kSqThreads) void atlas_sq_box(const float* __restrict__ atlas,
+                                                           long long N, int nx, int ny, int nz,
+                                                           int n_atlas, int zc, int n_chunks,
+                                                           float scale,
+                                                           float* __restrict__ aa) {
+    __shared__ float tile[kSqStep * kSqTile];
     const int ly = threadIdx.x % kSqBoxY, lx = threadIdx.x / kSqBoxY;
     const int y0 = blockIdx.x * kSqBoxY, x0 = blockIdx.y * kSqBoxX;
     const long long plane = (long long)nx * ny;
     const int y = y0 + ly, x = x0 + lx;
-    for (int at = blockIdx.z; at < n_atlas; at += gridDim.z) {
-    const float* __restrict__ A = atlas + (long long)at * N;
-    float* __restrict__ out = aa + (long long)at * N;
-    float s_prev = 0.f, s_cur = 0.f;   // in-plane sums of planes z-2 and z-1
-    for (int z = 0; z <= nz; ++z) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < (kSqBoxX + 2) * (kSqBoxY + 2); e += kSqBoxY * kSqBoxX) {
-            const int rx = e / (kSqBoxY + 2), cy = e - rx * (kSqBoxY + 2);
-            const int xx = x0 - 1 + rx, yy = y0 - 1 + cy;
-            float v = 0.f;
-            if (z < nz && xx >= 0 && xx < nx && yy >= 0 && yy < ny)
-                v = A[z * plane + (long long)xx * ny + yy] * scale;
-            tile[rx][cy] = v;
-        }
-        __syncthreads();
-        float s = 0.f;
+    // the lane's tile elements: in-plane offset, and whether they lie in the plane
+    long long off[kSqLoads];
+    bool in[kSqLoads];
 #pragma unroll
-        for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-            for (int dy = 0; dy < 3; ++dy) {
-                const float v = tile[lx + dx][ly + dy];
-                s = fmaf(v, v, s);
-            }
-        // plane z-1 is complete: its sum over planes z-2, z-1, z
-        if (z >= 1 && x < nx && y < ny)
-            out[(z - 1) * plane + (long long)x * ny + y] = (s_prev + s_cur) + s;
-        s_prev = s_cur;
-        s_cur = s;
+    for (int j = 0; j < kSqLoads; ++j) {
+        const int e = threadIdx.x + j * kSqThreads;
+        const int rx = e / (kSqBoxY + 2), cy = e - rx * (kSqBoxY + 2);
+        const int xx = x0 - 1 + rx, yy = y0 - 1 + cy;
+        in[j] = e < kSqTile && xx >= 0 && xx < nx && yy >= 0 && yy < ny;
+        off[j] = in[j] ? (long long)xx * ny + yy : 0;
     }
+    const int n_work = n_atlas * n_chunks;
+    for (int wi = blockIdx.z; wi < n_work; wi += gridDim.z) {
+        const int at = wi / n_chunks, ck = wi - at * n_chunks;
+        const int zs = ck * zc, ze = min(nz, zs + zc);   // output planes [zs, ze)
+        const float* __restrict__ A = atlas + (long long)at * N;
+        float* __restrict__ out = aa + (long long)at * N;
+        float s_prev = 0.f, s_cur = 0.f;   // in-plane sums of planes q-2 and q-1
+        // planes p .. p+3 of the lane's tile elements, scaled; zero outside
+        // the volume and past the chunk's last plane ze
+        float v[kSqStep][kSqLoads];
+        auto load_step = [&](int p) {
+#pragma unroll
+            for (int k = 0; k < kSqStep; ++k) {
+                const int q = p + k;
+                const bool qin = q >= 0 && q < nz && q <= ze;
+#pragma unroll
+                for (int j = 0; j < kSqLoads; ++j)
+                    v[k][j] = (qin && in[j]) ? A[q * plane + off[j]] * scale : 0.f;
+            }
+        };
+        load_step(zs - 1);
+        // planes zs-1 .. ze: plane q completes output plane q-1
+        for (int p = zs - 1; p <= ze; p += kSqStep) {
+            __syncthreads();   // the previous step's reads of the tiles are done
+#pragma unroll
+            for (int k = 0; k < kSqStep; ++k)
+#pragma unroll
+                for (int j = 0; j < kSqLoads; ++j) {
+                    const int e = threadIdx.x + j * kSqThreads;
+                    if (e < kSqTile) tile[k * kSqTile + e] = v[k][j];
+                }
+            __syncthreads();
+            // the next step's loads are in flight during this step's sums
+            if (p + kSqStep <= ze) load_step(p + kSqStep);
+#pragma unroll
+            for (int k = 0; k < kSqStep; ++k) {
+                const int q = p + k;
+                if (q > ze) break;
+                const float* t = tile + k * kSqTile;
+                float s = 0.f;
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                    for (int dy = 0; dy < 3; ++dy) {
+                        const float w = t[(lx + dx) * (kSqBoxY + 2) + ly + dy];
+                        s = fmaf(w, w, s);
+                    }
+                if (q - 1 >= zs && x < nx && y < ny)
+                    out[(q - 1) * plane + (long long)x * ny + y] = (s_prev + s_cur) + s;
+                s_prev = s_cur;
+                s_cur = s;
+            }
+        }
     }
 }
 
@@ -530,10 +574,18 @@ int launch_atlas_aa(const pct_problem* pb, const float* atlas, int n_atlas, floa
                     float* aa, hipStream_t stream) {
     if (n_atlas <= 0) return PCT_OK;
     const long long N = (long long)pb->nx * pb->ny * pb->nz;
-    const dim3 grid((pb->ny + kSqBoxY - 1) / kSqBoxY, (pb->nx + kSqBoxX - 1) / kSqBoxX,
-                    (unsigned)std::min(n_atlas, 65535));
-    hipLaunchKernelGGL(atlas_sq_box, grid, dim3(kSqBoxY * kSqBoxX), 0, stream, atlas, N, pb->nx,
-                       pb->ny, pb->nz, n_atlas, scale, aa);
+    const int gy = (pb->ny + kSqBoxY - 1) / kSqBoxY, gx = (pb->nx + kSqBoxX - 1) / kSqBoxX;
+    // z chunks: about 8192 workgroups (eight per CU, four waves each), chunks
+    // of at least kSqMinChunk planes
+    const long long base = (long long)gy * gx * n_atlas;
+    int n_chunks = (int)std::min<long long>((8192 + base - 1) / base,
+                                            std::max(1, pb->nz / kSqMinChunk));
+    n_chunks = std::max(1, n_chunks);
+    const int zc = (pb->nz + n_chunks - 1) / n_chunks;
+    n_chunks = (pb->nz + zc - 1) / zc;
+    const dim3 grid(gy, gx, (unsigned)std::min<long long>((long long)n_atlas * n_chunks, 65535));
+    hipLaunchKernelGGL(atlas_sq_box, grid, dim3(kSqThreads), 0, stream, atlas, N, pb->nx, pb->ny,
+                       pb->nz, n_atlas, zc, n_chunks, scale, aa);
     return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;
 }
 

@@ -25,7 +25,7 @@ SUBJECTS = [
     (10, 10, 10, 1, 11),
     (44, 40, 38, 4, 7),
 ]
-BATCH_SHAPES = [7, 8, 9, 10, 11, 13, 16, 17]
+BATCH_SHAPES = [7, 8, 9, 10, 11, 13, 16, 17, 19]
 
 
 def make_subjects(specs, seed=0):
