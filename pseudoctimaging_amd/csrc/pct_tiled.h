@@ -170,6 +170,27 @@ __device__ __forceinline__ void tiled_step(const float* __restrict__ lds, int ba
     tiled_step_at<C, BX, BZ>(lds, ba, bl, 0, tv, P, S);
 }
 
+// Staging loads read clamped positions (always inside the volume) and
+// replace the values of positions outside it after the load.  A load under
+// a branch would be waited for before the branch closes, one load at a
+// time; without one, a lane's loads of a plane batch are all in flight
+// together.
+__device__ __forceinline__ int clampi(int v, int n) { return min(max(v, 0), n - 1); }
+
+// The lane's slab columns c = lane + 64 cc: clamped volume y, and whether
+// the column is a slab column inside the volume.
+template <class C>
+__device__ __forceinline__ void lane_columns(int ys, int ny, int lane, int (&ycl)[C::COLCHUNKS],
+                                             bool (&yin)[C::COLCHUNKS]) {
+#pragma unroll
+    for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
+        const int c = lane + cc * 64;
+        const int yy = ys + c;
+        yin[cc] = c < C::CW && yy >= 0 && yy < ny;
+        ycl[cc] = clampi(yy, ny);
+    }
+}
+
 // Copies the slab of one (atlas, dz) stage into LDS: plane p < APLANES holds
 // the atlas MRI at z = zdz-1+p (scaled), plane APLANES+j the atlas CT at
 // z = zdz+j; row rr is volume x = xs+rr, column c is volume y = ys+c; zero
@@ -185,6 +206,9 @@ __device__ __forceinline__ void stage_slab(float* __restrict__ lds, const float*
     constexpr int RPW = (C::RX + NW - 1) / NW;   // rows per wave and plane
     constexpr int PB = 2;                         // planes per batch
     static_assert(NP % PB == 0, "plane batches must tile the slab");
+    int ycl[C::COLCHUNKS];
+    bool yin[C::COLCHUNKS];
+    lane_columns<C>(ys, ny, lane, ycl, yin);
 #pragma unroll 1
     for (int p0 = 0; p0 < NP; p0 += PB) {
         float v[PB][RPW][C::COLCHUNKS];
@@ -196,20 +220,17 @@ __device__ __forceinline__ void stage_slab(float* __restrict__ lds, const float*
             const float* __restrict__ src = is_mri ? A : L;
             const float mul = is_mri ? scale : 1.f;
             const bool zok = zz >= 0 && zz < nz;
-            const long long zoff = (long long)zz * plane;
+            const float* __restrict__ sz = src + (long long)clampi(zz, nz) * plane;
 #pragma unroll
             for (int k = 0; k < RPW; ++k) {
                 const int rr = wv + k * NW;
                 const int xx = xs + rr;
                 const bool xok = zok && rr < C::RX && xx >= 0 && xx < nx;
+                const float* __restrict__ sr = sz + (long long)clampi(xx, nx) * ny;
 #pragma unroll
                 for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
-                    const int c = lane + cc * 64;
-                    const int yy = ys + c;
-                    float t = 0.f;
-                    if (xok && c < C::CW && yy >= 0 && yy < ny)
-                        t = src[zoff + (long long)xx * ny + yy] * mul;
-                    v[q][k][cc] = t;
+                    const float raw = sr[ycl[cc]];
+                    v[q][k][cc] = (xok && yin[cc]) ? raw * mul : 0.f;
                 }
             }
         }
@@ -326,6 +347,17 @@ __device__ __forceinline__ void load_target_block(const float* __restrict__ img,
         }
 }
 
+// Whether wave wv of a workgroup has an output row before x_end.  A tile
+// column that runs past the interior (x_end = nx - r) holds waves whose rows
+// all lie beyond it: they stage their share of every slab and take part in
+// the barriers, but skip the candidates, whose sums would never be stored.
+// Uniform over the wave (read from the first lane), so the skip is a scalar
+// branch.
+template <class C, int BX>
+__device__ __forceinline__ bool wave_has_rows(int x0, int x_end, int wv) {
+    return x0 + __builtin_amdgcn_readfirstlane(wv) * C::S * BX < x_end;
+}
+
 // One work item (tile, atlas group) of one problem: the whole workgroup runs
 // it.  Shared by the single-problem kernel and the persistent batched kernel,
 // so a problem gives bit-identical results through either.
@@ -372,6 +404,7 @@ __device__ __forceinline__ void tiled_item(const TiledArgs& args, int item,
     const int ys = y0 - 1 - HW;   // volume y of LDS column 0
     const int a_begin = g * args.group;
     const int a_end = min(args.n_atlas, a_begin + args.group);
+    const bool live = wave_has_rows<C, BX>(x0, nx - r, wv);
 
     for (int a = a_begin; a < a_end; ++a) {
         const float* __restrict__ A = args.atlas + (long long)a * args.N;
@@ -380,6 +413,7 @@ __device__ __forceinline__ void tiled_item(const TiledArgs& args, int item,
             __syncthreads();
             stage_slab<C, NW>(lds, A, L, args.scale, z0 + dz, xs, ys, nx, ny, nz, plane, wv, lane);
             __syncthreads();
+            if (!live) continue;
             // (dx, dy) loops stay rolled: unrolling dy removes the per-step
             // base adds but multiplies the code size, and the batched graph
             // runs several of these kernels side by side
@@ -629,6 +663,9 @@ __device__ __forceinline__ int stage_slab_xt(float* __restrict__ lds, const floa
     constexpr int RPW = (C::RX + NW - 1) / NW;   // rows per wave and plane
     int out_of_range = 0;
     f32x2* __restrict__ lp = reinterpret_cast<f32x2*>(lds);
+    int ycl[C::COLCHUNKS];
+    bool yin[C::COLCHUNKS];
+    lane_columns<C>(ys, ny, lane, ycl, yin);
     // MRI: one pair plane (two volume planes) per round
 #pragma unroll 1
     for (int k = 0; k < X::MPAIRS; ++k) {
@@ -637,19 +674,17 @@ __device__ __forceinline__ int stage_slab_xt(float* __restrict__ lds, const floa
         for (int h = 0; h < 2; ++h) {
             const int zz = zdz - 1 + 2 * k + h;
             const bool zok = zz >= 0 && zz < nz;
-            const long long zoff = (long long)zz * plane;
+            const float* __restrict__ Az = A + (long long)clampi(zz, nz) * plane;
 #pragma unroll
             for (int r = 0; r < RPW; ++r) {
                 const int rr = wv + r * NW;
                 const int xx = xs + rr;
                 const bool xok = zok && rr < C::RX && xx >= 0 && xx < nx;
+                const float* __restrict__ Ar = Az + (long long)clampi(xx, nx) * ny;
 #pragma unroll
                 for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
-                    const int c = lane + cc * 64;
-                    const int yy = ys + c;
-                    float t = 0.f;
-                    if (xok && c < C::CW && yy >= 0 && yy < ny)
-                        t = A[zoff + (long long)xx * ny + yy] * scale;
+                    const float raw = Ar[ycl[cc]];
+                    const float t = (xok && yin[cc]) ? raw * scale : 0.f;
                     out_of_range |= !(fabsf(t) <= kXtBound);
                     v[r][cc][h] = t;
                 }
@@ -670,21 +705,21 @@ __device__ __forceinline__ int stage_slab_xt(float* __restrict__ lds, const floa
     for (int j = 0; j < C::LPLANES; ++j) {
         const int zz = zdz + j;
         const bool zok = zz >= 0 && zz < nz;
-        const long long zoff = (long long)zz * plane;
+        const long long zoff = (long long)clampi(zz, nz) * plane;
         float lv[RPW][C::COLCHUNKS], av[RPW][C::COLCHUNKS];
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
             const int rr = wv + r * NW;
             const int xx = xs + rr;
             const bool xok = zok && rr < C::RX && xx >= 0 && xx < nx;
+            const long long ro = zoff + (long long)clampi(xx, nx) * ny;
 #pragma unroll
             for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
-                const int c = lane + cc * 64;
-                const int yy = ys + c;
-                const bool in = xok && c < C::CW && yy >= 0 && yy < ny;
-                const long long o = zoff + (long long)xx * ny + yy;
-                lv[r][cc] = in ? L[o] : 0.f;
-                av[r][cc] = in ? AA[o] : __builtin_inff();
+                const bool in = xok && yin[cc];
+                const float lraw = L[ro + ycl[cc]];
+                const float araw = AA[ro + ycl[cc]];
+                lv[r][cc] = in ? lraw : 0.f;
+                av[r][cc] = in ? araw : __builtin_inff();
             }
         }
 #pragma unroll
@@ -713,23 +748,34 @@ __device__ __forceinline__ void restage_ct_raw(float* __restrict__ lds,
                                                int wv, int lane) {
     using X = XtCfg<C>;
     constexpr int RPW = (C::RX + NW - 1) / NW;
+    int ycl[C::COLCHUNKS];
+    bool yin[C::COLCHUNKS];
+    lane_columns<C>(ys, ny, lane, ycl, yin);
 #pragma unroll 1
     for (int j = 0; j < C::LPLANES; ++j) {
         const int zz = zdz + j;
         const bool zok = zz >= 0 && zz < nz;
+        const float* __restrict__ Lz = L + (long long)clampi(zz, nz) * plane;
+        float lv[RPW][C::COLCHUNKS];
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
             const int rr = wv + r * NW;
             const int xx = xs + rr;
             const bool xok = zok && rr < C::RX && xx >= 0 && xx < nx;
+            const float* __restrict__ Lr = Lz + (long long)clampi(xx, nx) * ny;
+#pragma unroll
+            for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
+                const float raw = Lr[ycl[cc]];
+                lv[r][cc] = (xok && yin[cc]) ? raw : 0.f;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int rr = wv + r * NW;
 #pragma unroll
             for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
                 const int c = lane + cc * 64;
-                const int yy = ys + c;
-                if (rr < C::RX && c < C::CW)
-                    lds[X::le(j, rr * C::CW + c)] =
-                        (xok && yy >= 0 && yy < ny) ? L[(long long)zz * plane + (long long)xx * ny + yy]
-                                                    : 0.f;
+                if (rr < C::RX && c < C::CW) lds[X::le(j, rr * C::CW + c)] = lv[r][cc];
             }
         }
     }
@@ -787,6 +833,8 @@ __device__ __forceinline__ void tiled_item_xt(const TiledArgs& args, int item,
     // 2^-TT per output (1 where the item keeps the squared differences)
     float f[BZ][BX];
 
+    const bool live = wave_has_rows<C, BX>(x0, nx - r, wv);
+
     if (squared_item) {
         for (int a = a_begin; a < a_end; ++a) {
             const float* __restrict__ A = args.atlas + (long long)a * args.N;
@@ -796,6 +844,7 @@ __device__ __forceinline__ void tiled_item_xt(const TiledArgs& args, int item,
                 stage_slab<C, NW>(lds, A, L, args.scale, z0 + dz, xs, ys, nx, ny, nz, plane, wv,
                                   lane);
                 __syncthreads();
+                if (!live) continue;
 #pragma unroll 1
                 for (int dx = 0; dx <= 2 * HW; ++dx) {
 #pragma unroll 1
@@ -836,6 +885,7 @@ __device__ __forceinline__ void tiled_item_xt(const TiledArgs& args, int item,
                 const int flag = stage_slab_xt<C, NW>(lds, A, L, AA, args.scale, z0 + dz, xs, ys,
                                                       nx, ny, nz, plane, wv, lane);
                 if (__syncthreads_or(flag) == 0) {
+                    if (!live) continue;
 #pragma unroll 1
                     for (int dx = 0; dx <= 2 * HW; ++dx) {
                         unsigned b[X::NB];
@@ -851,6 +901,7 @@ __device__ __forceinline__ void tiled_item_xt(const TiledArgs& args, int item,
                 } else {
                     restage_ct_raw<C, NW>(lds, L, z0 + dz, xs, ys, nx, ny, nz, plane, wv, lane);
                     __syncthreads();
+                    if (!live) continue;
 #pragma unroll 1
                     for (int dx = 0; dx <= 2 * HW; ++dx) {
 #pragma unroll 1
