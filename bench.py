@@ -360,6 +360,8 @@ def main() -> int:
     ap.add_argument("--beta", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tile-shape", type=int, default=-1, metavar="ID",
+                    help="run the plain leg with this tile shape (atlas group 1), no autotuning")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=48)
     ap.add_argument("--parity-voxels", type=int, default=2000)
@@ -428,7 +430,10 @@ def main() -> int:
     torch.cuda.synchronize()
 
     tuned = None
-    if not args.no_autotune:
+    if args.tile_shape >= 0:
+        pct.force_tile_config(args.tile_shape, 1)
+        log(f"tile shape {args.tile_shape} forced")
+    elif not args.no_autotune:
         log("autotuning tile configuration")
         tuned = pct.autotune(d_img, d_amri, d_lab, d_out, nh, sigma, beta, reps=1, stream=stream)
         log(f"tuned: {tuned}")
@@ -483,17 +488,20 @@ def main() -> int:
     prof = None
     if (nz, nx, ny, n, nh) == (112, 222, 222, 10, 11) and os.path.exists(tfile):
         with open(tfile) as f:
-            prof = json.load(f)
+            profiles = json.load(f)
+        kernels = profiles.get("kernels", {})
         why = None
-        if prof.get("kernel") != timed_kernel:
-            why = f"profiled kernel {prof.get('kernel')} is not the timed kernel {timed_kernel}"
-        elif prof.get("source_stamp") != stamp:
-            why = (f"profile taken of kernel sources {prof.get('source_stamp')}, built here "
+        if profiles.get("source_stamp") != stamp:
+            why = (f"profile taken of kernel sources {profiles.get('source_stamp')}, built here "
                    f"{stamp}")
+        elif timed_kernel not in kernels:
+            why = (f"the timed kernel {timed_kernel} is not among the profiled kernels "
+                   f"{sorted(kernels)}")
+        else:
+            prof = kernels[timed_kernel]
         if why:
-            # counters of another kernel do not describe this launch
+            # counters of another kernel (or other sources) do not describe this launch
             roofline["counters_note"] = why
-            prof = None
     if prof is not None:
         roofline["traffic"] = prof.get("hbm_bytes_per_launch")
         v = prof["valu"]
@@ -529,6 +537,8 @@ def main() -> int:
             f"post {e2e['post_host_ms']:.1f} ms); pct_pseudo_ct_host "
             f"{e2e['pseudo_ct_host_ms']:.1f} ms, same output {e2e['pseudo_ct_host_same_output']}")
 
+    if args.tile_shape >= 0:
+        pct.force_tile_config(-1, 0)   # the other legs choose their own shapes
     nan_fill = None
     if not args.no_nan_fill:
         log("NaN-fill leg: random-mode volume at sigma 0.33, device vs host post chain")
