@@ -363,7 +363,7 @@ def main() -> int:
     ap.add_argument("--tile-shape", type=int, default=-1, metavar="ID",
                     help="run the plain leg with this tile shape (atlas group 1), no autotuning")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=48)
+    ap.add_argument("--cpu-rows", type=int, default=96)
     ap.add_argument("--parity-voxels", type=int, default=2000)
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-pruned leg")

@@ -66,3 +66,43 @@ def test_deterministic(gpu):
     a = pct.multi_patch_based_segmentation_gpu(img, amri, lab, 9, 2.5, 1.0)
     b = pct.multi_patch_based_segmentation_gpu(img, amri, lab, 9, 2.5, 1.0)
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+# The cross-term shapes (tile configurations 15-19) weigh with
+# SSD' = TT + AA - C only where every scaled MRI value is at most 0.25 in
+# magnitude.  An item with a larger target value keeps the squared
+# differences throughout; a slab with a larger atlas value is weighed with
+# squared differences on the cross-term scale.  Each case below steers the
+# kernels into one of those paths, and every path must meet the same bar.
+XT_SHAPES = [15, 16, 17, 18, 19]
+
+
+def _xt_variants():
+    img, amri, act = pct.synth_tissue(40, 44, 30, 2, 2, False, 11)
+    lab = act.copy()
+    pct.normalize_labels(lab)
+    spiked_atlas = amri.copy()
+    spiked_atlas[1, 10:12, 20, 20:22] = 30.0     # scaled beyond the bound: slab fallback
+    spiked_img = img.copy()
+    spiked_img[12, 15, 18] = 25.0                 # a target value beyond it: squared items
+    uimg, uatlas, uct = pct.synth_uniform(40, 44, 30, 2, 12)   # values up to 10
+    ulab = uct.copy()
+    pct.normalize_labels(ulab)
+    return [("tissue", img, amri, lab), ("atlas-spike", img, spiked_atlas, lab),
+            ("target-spike", spiked_img, amri, lab), ("both", spiked_img, spiked_atlas, lab),
+            ("uniform", uimg, uatlas, ulab)]
+
+
+@pytest.mark.parametrize("nh,sigma", [(7, 2.5), (9, 1.0), (5, 0.5)])
+def test_cross_term_range_fallbacks(gpu, nh, sigma):
+    try:
+        for name, img, amri, lab in _xt_variants():
+            ref = oracle.fuse(img, amri, lab, nh, sigma, 1.0)
+            for sid in XT_SHAPES:
+                print(f"case {name} nh={nh} sigma={sigma} shape={sid}", flush=True)
+                pct.force_tile_config(sid, 1)
+                got = pct.multi_patch_based_segmentation_gpu(img, amri, lab, nh, sigma, 1.0,
+                                                             pct.CHOICE_SM)
+                compare(got, ref)
+    finally:
+        pct.force_tile_config(-1, 0)
