@@ -53,7 +53,7 @@ METRIC = ("pseudo-CT voxels/s @222\u00d7222\u00d7112 n=10 NH=11: plain vs learne
 # the files whose contents define the tiled fusion kernels: a profile is
 # attached to a run only when it was taken of a build of the same contents
 KERNEL_SOURCES = ("pseudoctimaging_amd/csrc/pct_tiled.h", "pseudoctimaging_amd/csrc/pct_fuse.hip",
-                  "pseudoctimaging_amd/csrc/Makefile")
+                  "pseudoctimaging_amd/csrc/pct_learned.hip", "pseudoctimaging_amd/csrc/Makefile")
 PROFILE = os.path.join("profiles", "fuse_tiled_222x222x112_n10_nh11_traffic.json")
 VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4   # wave64 FP32 instructions, non-packed
 
@@ -64,6 +64,39 @@ def kernel_source_stamp() -> str:
         with open(os.path.join(ROOT, rel), "rb") as f:
             h.update(rel.encode() + b"\0" + f.read())
     return h.hexdigest()[:16]
+
+
+def profiled_counters(kernel: str):
+    """The committed counter profile of `kernel` at 222x222x112, n=10, NH=11,
+    and a note: (entry, None) when the profile was taken of these kernel
+    sources and holds that kernel, else (None, why not)."""
+    tfile = os.path.join(ROOT, PROFILE)
+    if not os.path.exists(tfile):
+        return None, f"no profile file {PROFILE}"
+    with open(tfile) as f:
+        profiles = json.load(f)
+    stamp = kernel_source_stamp()
+    kernels = profiles.get("kernels", {})
+    if profiles.get("source_stamp") != stamp:
+        return None, (f"profile taken of kernel sources {profiles.get('source_stamp')}, built "
+                      f"here {stamp}")
+    if kernel not in kernels:
+        return None, f"the timed kernel {kernel} is not among the profiled kernels {sorted(kernels)}"
+    return kernels[kernel], None
+
+
+def counter_roofline(entry, kernel_ms):
+    """achieved / frac / issue_frac of one launch of `kernel_ms` from a
+    profile entry's VALU counters."""
+    v = entry["valu"]
+    executed = v["executed_fp32_flop_per_launch"] / (kernel_ms * 1e-3) / 1e12
+    return {"achieved": round(executed, 3), "frac": round(executed / PEAK_FP32_TFLOPS, 4),
+            "issue_frac": round((v["SQ_INSTS_VALU"] - v["SQ_INSTS_VALU_TRANS_F32"])
+                                / (kernel_ms * 1e-3) / VALU_ISSUE_PER_S, 4)}
+
+
+# tile shape of the pruned kernel by shape id: (W, BX, BZ, waves)
+PRUNED_SHAPES = {13: (32, 2, 4, 8), 8: (32, 4, 2, 4)}
 
 
 def host_cpu_info() -> dict:
@@ -228,6 +261,9 @@ def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup,
     for _ in range(warmup):
         learned.fuse_pruned_device(d_img, d_amri, d_lab, out, nh, sigma, beta, params,
                                    report_fallback=False, **kw)
+    from pseudoctimaging_amd import _lib
+    L = _lib.lib()
+    L.pct_kernel_timing(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -235,6 +271,23 @@ def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup,
                                    report_fallback=False, **kw)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
+    tot_ms, launches = _lib.c_double(), _lib.c_int()
+    _lib.check(L.pct_kernel_timing_read(_lib.ctypes.byref(tot_ms), _lib.ctypes.byref(launches)))
+    L.pct_kernel_timing(0)
+    kernel_ms = tot_ms.value / max(1, launches.value)
+    w, bx, bz, nw = PRUNED_SHAPES[shape_id if shape_id >= 0 else 13]
+    kname = f"pruned_kernel<{w},{bx},{bz},{nw},{(nh - 1) // 2}>"
+    roofline = {"bound": "valu_fp32", "kernel": kname, "kernel_ms": round(kernel_ms, 4),
+                "launches": launches.value, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "achieved": None, "frac": None, "issue_frac": None}
+    nz_, nx_, ny_ = plain_out.shape
+    if (nz_, nx_, ny_, n, nh, topk, samples, max_slabs) == (112, 222, 222, 10, 11, 32, 27, 3):
+        entry, why = profiled_counters(kname)
+        if entry is not None:
+            roofline.update(counter_roofline(entry, kernel_ms))
+            roofline["counters_source"] = PROFILE
+        else:
+            roofline["counters_note"] = why
     a = out.cpu().numpy().astype(np.float64)
     b = plain_out.cpu().numpy().astype(np.float64)
     ok = np.isfinite(a) & np.isfinite(b)
@@ -253,7 +306,8 @@ def learned_leg(d_img, d_amri, d_lab, plain_out, nh, sigma, beta, steps, warmup,
                       "anchors_per_step": 256, "seconds": train_s,
                       "loss_first_last": losses},
             "timed": "embeddings of target and atlases + selection + pruned weighting "
-                     "+ fallback + finalize, per call"}
+                     "+ fallback + finalize, per call",
+            "pruned_roofline": roofline}
 
 
 def nan_fill_leg(seed, reps=3):
@@ -484,33 +538,15 @@ def main() -> int:
         "algorithmic_bytes": 4 * full * (2 + 2 * n),
         "source_stamp": stamp,
     }
-    tfile = os.path.join(ROOT, PROFILE)
-    prof = None
-    if (nz, nx, ny, n, nh) == (112, 222, 222, 10, 11) and os.path.exists(tfile):
-        with open(tfile) as f:
-            profiles = json.load(f)
-        kernels = profiles.get("kernels", {})
-        why = None
-        if profiles.get("source_stamp") != stamp:
-            why = (f"profile taken of kernel sources {profiles.get('source_stamp')}, built here "
-                   f"{stamp}")
-        elif timed_kernel not in kernels:
-            why = (f"the timed kernel {timed_kernel} is not among the profiled kernels "
-                   f"{sorted(kernels)}")
+    if (nz, nx, ny, n, nh) == (112, 222, 222, 10, 11):
+        # counters of another kernel (or of other sources) do not describe this launch
+        prof, why = profiled_counters(timed_kernel)
+        if prof is not None:
+            roofline["traffic"] = prof.get("hbm_bytes_per_launch")
+            roofline.update(counter_roofline(prof, kernel_ms))
+            roofline["counters_source"] = PROFILE
         else:
-            prof = kernels[timed_kernel]
-        if why:
-            # counters of another kernel (or other sources) do not describe this launch
             roofline["counters_note"] = why
-    if prof is not None:
-        roofline["traffic"] = prof.get("hbm_bytes_per_launch")
-        v = prof["valu"]
-        executed = v["executed_fp32_flop_per_launch"] / (kernel_ms * 1e-3) / 1e12
-        roofline["achieved"] = round(executed, 3)
-        roofline["frac"] = round(executed / PEAK_FP32_TFLOPS, 4)
-        roofline["issue_frac"] = round((v["SQ_INSTS_VALU"] - v["SQ_INSTS_VALU_TRANS_F32"])
-                                       / (kernel_ms * 1e-3) / VALU_ISSUE_PER_S, 4)
-        roofline["counters_source"] = PROFILE
 
     out = d_out.cpu().numpy()
     parity = None
@@ -555,7 +591,9 @@ def main() -> int:
         learned_res.pop("params")
         log(f"learned: {learned_res['ms_per_step']:.2f} ms/step, speedup "
             f"{learned_res['speedup_vs_plain']:.2f}x, MAE {learned_res['mae_hu_vs_plain']:.3f} HU, "
-            f"fallback {learned_res['fallback_frac']:.3f}")
+            f"fallback {learned_res['fallback_frac']:.3f}; pruned kernel "
+            f"{learned_res['pruned_roofline']['kernel_ms']:.3f} ms, frac "
+            f"{learned_res['pruned_roofline']['frac']}")
 
     batched = None
     if not args.no_batch:

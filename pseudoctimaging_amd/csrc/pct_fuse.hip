@@ -651,8 +651,8 @@ int launch_tiled_items(const TiledArgs& a, int shape_id, int hw, hipStream_t str
     return PCT_OK;
 }
 
-// Optional per-launch HIP events around the fusion kernels (see
-// pct_kernel_timing).  Events are created once and reused.
+// Optional per-launch HIP events around the fusion kernels and the pruned
+// kernel (see pct_kernel_timing).  Events are created once and reused.
 struct KernelTimer {
     std::mutex mu;
     bool enabled = false;
@@ -675,16 +675,16 @@ struct KernelTimer {
 };
 static KernelTimer g_timer;
 
-struct TimedLaunch {
-    std::pair<hipEvent_t, hipEvent_t> ev;
-    hipStream_t s;
-    explicit TimedLaunch(hipStream_t st) : ev(g_timer.next()), s(st) {
-        if (ev.first) (void)hipEventRecord(ev.first, s);
-    }
-    ~TimedLaunch() {
-        if (ev.second) (void)hipEventRecord(ev.second, s);
-    }
-};
+TimedLaunch::TimedLaunch(hipStream_t st) : s(st) {
+    const auto ev = g_timer.next();
+    begin = ev.first;
+    end = ev.second;
+    if (begin) (void)hipEventRecord(begin, s);
+}
+
+TimedLaunch::~TimedLaunch() {
+    if (end) (void)hipEventRecord(end, s);
+}
 
 static int launch_tiled(const pct_problem* pb, const float* img, const float* atlas,
                         const float* label, float* out, void* ws, size_t ws_bytes,

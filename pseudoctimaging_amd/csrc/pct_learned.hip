@@ -1833,6 +1833,7 @@ extern "C" int pct_fuse_pruned_dev(const pct_problem* pb, const float* params,
                               hipFuncAttributeMaxDynamicSharedMemorySize, g.lds_bytes);
     if (items > 0) {
         void* kargs[] = {&pa};
+        TimedLaunch timed(s);
         if (hipLaunchKernel(reinterpret_cast<const void*>(fn), dim3((unsigned)items),
                             dim3(sh.waves * 64), kargs, (size_t)g.lds_bytes, s) != hipSuccess)
             return PCT_ERR_HIP;

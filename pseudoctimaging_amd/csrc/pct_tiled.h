@@ -1002,6 +1002,18 @@ int launch_finalize(const pct_problem* pb, const TileGeometry& g, const float* w
 // `a` (honouring a.only), without the finalize pass.
 int launch_tiled_items(const TiledArgs& a, int shape_id, int hw, hipStream_t stream);
 
+// Records a pair of HIP events on `s` around one kernel launch (the scope of
+// the object) while kernel timing is on (pct_kernel_timing); a no-op
+// otherwise.
+struct TimedLaunch {
+    hipEvent_t begin = nullptr, end = nullptr;
+    hipStream_t s;
+    explicit TimedLaunch(hipStream_t st);
+    ~TimedLaunch();
+    TimedLaunch(const TimedLaunch&) = delete;
+    TimedLaunch& operator=(const TimedLaunch&) = delete;
+};
+
 // Enqueues the direct kernel (any window).
 int launch_direct(const pct_problem* pb, const float* img, const float* atlas,
                   const float* label, float* out, hipStream_t stream);
