@@ -134,19 +134,48 @@ __global__ __launch_bounds__(kListThreads) void nan_write_kernel(const float* __
 
 constexpr int kFillThreads = 1024;
 
-// Median of the non-NaN values among the 27 neighbourhood values of `id`
-// (cpp:1020-1042): the element(s) of rank cnt/2 (and cnt/2 - 1) in a stable
-// ascending order, the even-count mean in double; NaN without a value.
-__device__ __forceinline__ float neighbourhood_median(const float* arr, int id, int pl, int ny) {
-    float v[27];
-#pragma unroll
-    for (int k = 0; k < 27; ++k)
-        v[k] = __hip_atomic_load(arr + id + (k / 9 - 1) * pl + ((k / 3) % 3 - 1) * ny + (k % 3 - 1),
-                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    int cnt = 0;
-#pragma unroll
-    for (int k = 0; k < 27; ++k) cnt += isnan(v[k]) ? 0 : 1;
-    if (cnt == 0) return __builtin_nanf("");
+// Comparators of Batcher's merge-exchange sorting network for 27 inputs
+// (Knuth, TAOCP vol. 3, 5.2.2 Algorithm M; valid for any input count):
+// 155 (min, max) pairs, built at compile time.
+struct SortNet27 {
+    int n = 0;
+    unsigned char lo[160] = {}, hi[160] = {};
+};
+constexpr SortNet27 merge_exchange27() {
+    SortNet27 L{};
+    constexpr int N = 27;
+    int t = 0;
+    while ((1 << t) < N) ++t;
+    for (int p = 1 << (t - 1); p > 0; p >>= 1) {
+        int q = 1 << (t - 1), r = 0, d = p;
+        for (;;) {
+            for (int i = 0; i < N - d; ++i)
+                if ((i & p) == r) {
+                    L.lo[L.n] = (unsigned char)i;
+                    L.hi[L.n] = (unsigned char)(i + d);
+                    ++L.n;
+                }
+            if (q == p) break;
+            d = q - p;
+            q >>= 1;
+            r = p;
+        }
+    }
+    return L;
+}
+constexpr SortNet27 kSortNet27 = merge_exchange27();
+static_assert(kSortNet27.n == 155, "merge exchange for 27 inputs");
+
+// Order-preserving integer key of a float that is not NaN (an involution:
+// applied to a key it gives the float's bits back).  -0 sorts before +0.
+__device__ __forceinline__ int float_key(float f) {
+    const int u = __float_as_int(f);
+    return u ^ (int)((unsigned)(u >> 31) >> 1);
+}
+
+// Median of the non-NaN values among `cnt` of v[27] by stable ranks: the
+// std::stable_sort of cpp:1035, equal values in their gathered order.
+__device__ __forceinline__ float median_by_ranks(const float (&v)[27], int cnt) {
     const int hi = cnt / 2, lo_rank = cnt / 2 - 1;
     float a = 0.f, b = 0.f;
 #pragma unroll
@@ -163,10 +192,62 @@ __device__ __forceinline__ float neighbourhood_median(const float* arr, int id, 
     return b;
 }
 
+// Median of the 3x3x3 neighbourhood of voxel `id` over its non-NaN values
+// (cpp:1020-1043): the middle value, or the float64 mean of the two middle
+// values, of their stable sort; NaN when all 27 are NaN.  Values that compare
+// equal have the same bits except -0 and +0, so unless both signed zeros
+// occur, any sort of the values gives the stable sort's bits: a sorting
+// network of integer min/max pairs over order-preserving keys (NaN keyed
+// last), 155 pairs against 27 x 27 rank comparisons.  With both signed zeros
+// the ranks decide.
+__device__ __forceinline__ float neighbourhood_median(const float* arr, int id, int pl, int ny) {
+    float v[27];
+#pragma unroll
+    for (int k = 0; k < 27; ++k)
+        v[k] = __hip_atomic_load(arr + id + (k / 9 - 1) * pl + ((k / 3) % 3 - 1) * ny + (k % 3 - 1),
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    int cnt = 0;
+    bool neg_zero = false, pos_zero = false;
+    int key[27];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+        const bool nan = isnan(v[k]);
+        cnt += nan ? 0 : 1;
+        const int u = __float_as_int(v[k]);
+        neg_zero |= u == (int)0x80000000;
+        pos_zero |= u == 0;
+        key[k] = nan ? INT_MAX : float_key(v[k]);
+    }
+    if (cnt == 0) return __builtin_nanf("");
+    if (neg_zero && pos_zero) return median_by_ranks(v, cnt);
+#pragma unroll
+    for (int c = 0; c < kSortNet27.n; ++c) {
+        const int i = kSortNet27.lo[c], j = kSortNet27.hi[c];
+        const int mn = min(key[i], key[j]);
+        key[j] = max(key[i], key[j]);
+        key[i] = mn;
+    }
+    const int hi = cnt / 2, lo_rank = cnt / 2 - 1;
+    int ka = 0, kb = 0;
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+        kb = k == hi ? key[k] : kb;
+        ka = k == lo_rank ? key[k] : ka;
+    }
+    const float a = __int_as_float(float_key(__int_as_float(ka))), b = __int_as_float(float_key(__int_as_float(kb)));
+    if (cnt % 2 == 0) return (float)(((double)a + (double)b) / 2);
+    return b;
+}
+
 // The sweeps of RegularizationStep over the listed voxels, in one workgroup.
 // hist has t_count + 1 ints; sorted and the lists hold `interior` ints.
-// Values and list entries go through agent-scope atomics, and every level
-// ends with a fence and a barrier, so a level sees the ones before it.
+// Every thread that reads or writes these buffers during the fill belongs to
+// this one workgroup (the inputs come from earlier launches on the stream),
+// so values and list entries go through workgroup-scope atomics and every
+// level ends with a workgroup barrier, whose workgroup-scope release and
+// acquire let a level see the ones before it.  Agent-scope fences would
+// write back and invalidate the L2 at every level.  The level histogram is
+// built by atomics performed in L2 and read back with agent-scope loads.
 __global__ __launch_bounds__(kFillThreads) void nan_fill_levels_kernel(
     float* __restrict__ arr, int* list_a, int* list_b, int* sorted, int* hist,
     const int* count0, int nx, int ny, int t_count, int* sweeps_out) {
@@ -175,7 +256,7 @@ __global__ __launch_bounds__(kFillThreads) void nan_fill_levels_kernel(
     const int pl = nx * ny;
     int* cur = list_a;
     int* nxt = list_b;
-    if (threadIdx.x == 0) s_n = __hip_atomic_load(count0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) s_n = __hip_atomic_load(count0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __syncthreads();
     int n = s_n;
     int sweeps = 0;
@@ -183,34 +264,41 @@ __global__ __launch_bounds__(kFillThreads) void nan_fill_levels_kernel(
         ++sweeps;
         // counting sort of the list by t = 4 z + 2 x + y
         for (int t = threadIdx.x; t <= t_count; t += kFillThreads) hist[t] = 0;
-        __threadfence();
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += kFillThreads) {
             const int id = cur[i];
             const int z = id / pl, x = (id - z * pl) / ny, y = id - z * pl - x * ny;
             atomicAdd(hist + 4 * z + 2 * x + y, 1);
         }
-        __threadfence();
         __syncthreads();
         if (threadIdx.x == 0) {
-            // exclusive prefix of the counts; hist[t] becomes the level start
+            // exclusive prefix of the counts; hist[t] becomes the level start.
+            // The counts come from atomics performed in L2: read them with
+            // agent-scope loads, which do not use the CU's L1, 16 at a time.
             int run = 0;
-            for (int t = 0; t <= t_count; ++t) {
-                const int c = hist[t];
-                hist[t] = run;
-                run += c;
+            for (int t0 = 0; t0 <= t_count; t0 += 16) {
+                int c[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k)
+                    c[k] = t0 + k <= t_count ? __hip_atomic_load(hist + t0 + k, __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_AGENT)
+                                             : 0;
+#pragma unroll
+                for (int k = 0; k < 16; ++k)
+                    if (t0 + k <= t_count) {
+                        hist[t0 + k] = run;
+                        run += c[k];
+                    }
             }
             s_next = 0;
             s_filled = 0;
         }
-        __threadfence();
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += kFillThreads) {
             const int id = cur[i];
             const int z = id / pl, x = (id - z * pl) / ny, y = id - z * pl - x * ny;
             sorted[atomicAdd(hist + 4 * z + 2 * x + y, 1)] = id;
         }
-        __threadfence();
         __syncthreads();
         // hist[t] is now the end of level t; levels in increasing t, their
         // ends read into LDS a chunk at a time
@@ -218,7 +306,7 @@ __global__ __launch_bounds__(kFillThreads) void nan_fill_levels_kernel(
         for (int t0 = 0; t0 < t_count; t0 += kFillThreads) {
             if (t0 + (int)threadIdx.x < t_count)
                 s_end[threadIdx.x] = __hip_atomic_load(hist + t0 + threadIdx.x, __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT);
+                                                       __HIP_MEMORY_SCOPE_AGENT);   // L2 atomics
             __syncthreads();
             const int tn = min(kFillThreads, t_count - t0);
             for (int k = 0; k < tn; ++k) {
@@ -226,16 +314,15 @@ __global__ __launch_bounds__(kFillThreads) void nan_fill_levels_kernel(
                 if (end == begin) continue;   // uniform: an empty level
                 for (int i = begin + (int)threadIdx.x; i < end; i += kFillThreads) {
                     const int id = __hip_atomic_load(sorted + i, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
                     const float res = neighbourhood_median(arr, id, pl, ny);
-                    __hip_atomic_store(arr + id, res, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(arr + id, res, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     if (isnan(res))
                         __hip_atomic_store(nxt + atomicAdd(&s_next, 1), id, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
                     else
                         atomicAdd(&s_filled, 1);
                 }
-                __threadfence();
                 __syncthreads();
                 begin = end;
             }
