@@ -15,10 +15,13 @@ Fixtures
   small_uniform.npz   random-mode data (10*U[0,1)), 14x15x13, n=2, NH=3,
                       sigma 0.33, beta 1: a mix of NaN and finite voxels, so the
                       NaN rule and the median fill are exercised.
-  cfg1.json           64x64x32, n=2, NH=7 (the CPU-sized case): input and
+  full_64x64x32_n2_nh7.json
+                      64x64x32, n=2, NH=7 (the CPU-sized case): input and
                       output sha256 plus 2048 sampled voxel values.
-  cfg2_sample.npz     222x222x112, n=5, NH=9 and
-  cfg3_sample.npz     222x222x112, n=10, NH=11: input sha256, sampled voxel
+  sample_222x222x112_n5_nh9.npz
+                      222x222x112, n=5, NH=9, and
+  sample_222x222x112_n10_nh11.npz
+                      222x222x112, n=10, NH=11: input sha256, sampled voxel
                       indices, the reference's float32 value at each (computed
                       on crops that contain the voxel's whole neighbourhood)
                       and the float64 oracle value P/S.
@@ -90,7 +93,7 @@ def small_uniform():
     print("small_uniform: NaN", int(np.isnan(inner).sum()), "of", inner.size)
 
 
-def cfg1():
+def full_64x64x32():
     nz, nx, ny, n, nh, sigma, beta, seed = 32, 64, 64, 2, 7, 2.5, 1.0, 1
     img, amri, act = pct.synth_tissue(nx, ny, nz, n, 2, False, seed)
     lab, fused, hu = ref_pipeline(img, amri, act, nh, sigma, beta)
@@ -105,9 +108,9 @@ def cfg1():
         "sample_fused": [float(v) for v in fused.reshape(-1)[idx]],
         "sample_hu": [float(v) for v in hu.reshape(-1)[idx]],
     }
-    with open(os.path.join(HERE, "cfg1.json"), "w") as f:
+    with open(os.path.join(HERE, "full_64x64x32_n2_nh7.json"), "w") as f:
         json.dump(out, f)
-    print("cfg1 written")
+    print("full_64x64x32_n2_nh7.json written")
 
 
 def sampled(name, nz, nx, ny, n, nh, seed, centres=96, block=4):
@@ -149,6 +152,6 @@ if __name__ == "__main__":
         sys.exit("oracle/_ref/libpctref.so is required (make -C oracle)")
     small_tissue()
     small_uniform()
-    cfg1()
-    sampled("cfg2", 112, 222, 222, 5, 9, 2)
-    sampled("cfg3", 112, 222, 222, 10, 11, 3)
+    full_64x64x32()
+    sampled("sample_222x222x112_n5_nh9", 112, 222, 222, 5, 9, 2)
+    sampled("sample_222x222x112_n10_nh11", 112, 222, 222, 10, 11, 3)

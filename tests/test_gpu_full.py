@@ -38,7 +38,7 @@ def load(name):
     return dict(np.load(os.path.join(GOLDEN, name)))
 
 
-@pytest.mark.parametrize("name", ["cfg2_sample.npz", "cfg3_sample.npz"])
+@pytest.mark.parametrize("name", ["sample_222x222x112_n5_nh9.npz", "sample_222x222x112_n10_nh11.npz"])
 def test_full_size_against_reference_samples(gpu, name):
     g = load(name)
     nx, ny, nz, n, nh = (int(v) for v in g["params"][:5])
@@ -67,8 +67,8 @@ def test_full_size_against_reference_samples(gpu, name):
     assert np.array_equal(out.view(np.uint32), again.view(np.uint32))
 
 
-def test_cfg3_atlas_order_invariance(gpu):
-    g = load("cfg3_sample.npz")
+def test_222x222x112_atlas_order_invariance(gpu):
+    g = load("sample_222x222x112_n10_nh11.npz")
     nx, ny, nz, n, nh = (int(v) for v in g["params"][:5])
     img, amri, act = pct.synth_tissue(nx, ny, nz, n, 3, False, int(g["params"][7]))
     lab = act.copy()
@@ -82,10 +82,10 @@ def test_cfg3_atlas_order_invariance(gpu):
     np.testing.assert_allclose(b[inner], a[inner], rtol=2e-6)
 
 
-def test_cfg3_linear_in_ct(gpu):
+def test_222x222x112_linear_in_ct(gpu):
     # The weights depend on the MRI only, so out(a L + b) = a out(L) + b
     # holds for the whole 222x222x112, n=10, NH=11 volume.
-    g = load("cfg3_sample.npz")
+    g = load("sample_222x222x112_n10_nh11.npz")
     nx, ny, nz, n, nh = (int(v) for v in g["params"][:5])
     img, amri, act = pct.synth_tissue(nx, ny, nz, n, 3, False, int(g["params"][7]))
     lab = act.copy()
@@ -102,10 +102,10 @@ def test_cfg3_linear_in_ct(gpu):
     assert not b[0].any() and not b[:, :, ny - 1].any()
 
 
-def test_cfg3_constant_mri_is_window_mean(gpu):
+def test_222x222x112_constant_mri_is_window_mean(gpu):
     # A constant MRI makes every weight exp(0) = 1: the fusion is the mean of
     # the atlas CT over the NH^3 window, at the full 222x222x112, n=10 size.
-    g = load("cfg3_sample.npz")
+    g = load("sample_222x222x112_n10_nh11.npz")
     nx, ny, nz, n, nh = (int(v) for v in g["params"][:5])
     _, _, act = pct.synth_tissue(nx, ny, nz, n, 3, False, int(g["params"][7]))
     lab = act.copy()
@@ -138,8 +138,8 @@ def test_cfg3_constant_mri_is_window_mean(gpu):
     assert rel.mean() < 2.0 ** -25 * 1331, rel.mean()
 
 
-def test_cfg1_full_volume(gpu):
-    with open(os.path.join(GOLDEN, "cfg1.json")) as f:
+def test_64x64x32_full_volume(gpu):
+    with open(os.path.join(GOLDEN, "full_64x64x32_n2_nh7.json")) as f:
         g = json.load(f)
     nz, nx, ny = g["shape_zxy"]
     img, amri, act = pct.synth_tissue(nx, ny, nz, g["n_atlas"], g["max_shift"], False, g["seed"])

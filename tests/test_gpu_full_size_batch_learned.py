@@ -6,7 +6,8 @@
   window class's tile shape and atlas group bit for bit, and sampled voxels
   match the FP64 oracle within the plain path's tolerance.
 * The learned-pruned path at 222x222x112, n=10, NH=11 (the inputs of the
-  cfg3 golden fixture, whose samples the plain leg is checked against), the
+  sample_222x222x112_n10_nh11 golden fixture, whose samples the plain leg is
+  checked against), the
   embedding trained on 4 other seeded phantoms: mean |dHU| against the plain
   output at most 5 HU, the NaN pattern identical, the fallback fraction a
   fraction.
@@ -28,12 +29,12 @@ BATCH = [(160, 160, 96, 1, 7), (192, 192, 96, 3, 9), (222, 222, 112, 5, 11),
          (222, 222, 112, 7, 11), (256, 256, 128, 10, 9), (128, 128, 64, 10, 7)]
 
 
-def test_batch_baseline_cfg4(gpu):
+def test_batch_six_uneven_subjects_full_size(gpu):
     import torch
     dev = torch.device("cuda:0")
     host, subs = [], []
     for k, (nx, ny, nz, n, nh) in enumerate(BATCH):
-        print(f"case cfg4 subject {k}: {nx}x{ny}x{nz} n={n} NH={nh}", flush=True)
+        print(f"case six-subject batch, subject {k}: {nx}x{ny}x{nz} n={n} NH={nh}", flush=True)
         img, amri, act = pct.synth_tissue(nx, ny, nz, n, 3, False, 500 + k)
         lab = act.copy()
         pct.normalize_labels(lab)
@@ -70,14 +71,14 @@ def test_batch_baseline_cfg4(gpu):
         pct.force_tile_config(-1, 0)
 
 
-def test_learned_baseline_cfg5(gpu):
+def test_learned_222x222x112_n10_nh11(gpu):
     import torch
-    g = dict(np.load(os.path.join(GOLDEN, "cfg3_sample.npz")))
+    g = dict(np.load(os.path.join(GOLDEN, "sample_222x222x112_n10_nh11.npz")))
     nx, ny, nz, n, nh = (int(v) for v in g["params"][:5])
     sigma, beta, seed, shift = (float(g["params"][5]), float(g["params"][6]),
                                 int(g["params"][7]), int(g["params"][8]))
     assert (nx, ny, nz, n, nh) == (222, 222, 112, 10, 11)
-    print(f"case cfg5: {nx}x{ny}x{nz} n={n} NH={nh}", flush=True)
+    print(f"case learned 222x222x112 n=10 NH=11: {nx}x{ny}x{nz} n={n} NH={nh}", flush=True)
     dev = torch.device("cuda:0")
     img, amri, act = pct.synth_tissue(nx, ny, nz, n, shift, False, seed)
     lab = act.copy()

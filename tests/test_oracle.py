@@ -61,8 +61,8 @@ def test_small_uniform_has_nan_and_finite():
     assert not np.isnan(g["hu"]).any()
 
 
-def test_oracle_matches_golden_cfg1():
-    with open(os.path.join(GOLDEN, "cfg1.json")) as f:
+def test_oracle_matches_golden_64x64x32():
+    with open(os.path.join(GOLDEN, "full_64x64x32_n2_nh7.json")) as f:
         g = json.load(f)
     nz, nx, ny = g["shape_zxy"]
     img, amri, act = pct.synth_tissue(nx, ny, nz, g["n_atlas"], g["max_shift"], False, g["seed"])
@@ -77,7 +77,7 @@ def test_oracle_matches_golden_cfg1():
     assert digest(hu) == g["hu_sha256"]
 
 
-@pytest.mark.parametrize("name", ["cfg2_sample.npz", "cfg3_sample.npz"])
+@pytest.mark.parametrize("name", ["sample_222x222x112_n5_nh9.npz", "sample_222x222x112_n10_nh11.npz"])
 def test_oracle_matches_golden_full_size_samples(name):
     g = load(name)
     nx, ny, nz, n, nh = (int(v) for v in g["params"][:5])
