@@ -1328,8 +1328,20 @@ __global__ __launch_bounds__(kMergeWaves * 64) void merge_offsets_kernel(MergeAr
         const int del = ((o / nw1) % nw1) * a.cw + o % nw1;
         return (int)((smpl << 31) | ((unsigned)(dz + 8) << 16) | (unsigned)del);
     };
+    // the kept offsets of a dz plane in (dy, dx) order, the order in which
+    // the plain kernel sweeps a slab's candidates (dy outer, dx inner), so
+    // that evaluating every candidate sums in the plain kernel's order
+    // (every lane runs the shuffles; only lanes with a kept offset use the rank)
+    int rank = 0;
+    const int kdz = kv / nw2, key = (kv % nw1) * nw1 + (kv / nw1) % nw1;
+    for (int q = 0; q < a.topk; ++q) {
+        const int o = __shfl(kv, q);
+        const int odz = o / nw2;
+        rank += o < kNoOffset &&
+                (odz < kdz || (odz == kdz && (o % nw1) * nw1 + (o / nw1) % nw1 < key));
+    }
     int* out = a.merged + item * kMergedStride;
-    if (has_k) out[1 + lane + before_k] = pack(kv, 0u);
+    if (has_k) out[1 + rank + before_k] = pack(kv, 0u);
     if (has_s && !dup)
         out[1 + before_s + __popcll(smask & ((1ull << lane) - 1ull))] = pack(sv, 1u);
     if (lane == 0) {

@@ -87,6 +87,12 @@ struct TiledCfg {
     static_assert(PPB >= 1, "one plane must fit an LDS immediate window");
     static constexpr int NBA = (APLANES + PPB - 1) / PPB;   // base registers, MRI planes
     static constexpr int NBL = (LPLANES + PPB - 1) / PPB;   // base registers, CT planes
+    // the same for the dx sweep, whose immediates carry the row (up to
+    // 2 HW + BX + 1) and no dy
+    static constexpr int PPBS = (16383 - (2 * HW + BX + 1) * CW) / PS + 1;
+    static_assert(PPBS >= 1, "one plane must fit an LDS immediate window");
+    static constexpr int NBAS = (APLANES + PPBS - 1) / PPBS;
+    static constexpr int NBLS = (LPLANES + PPBS - 1) / PPBS;
     static_assert(64 % W == 0, "segment width must divide the wavefront");
     static_assert(BX % 2 == 0 && BZ % 2 == 0, "pairwise box sums need even blocks");
 };
@@ -99,75 +105,77 @@ __device__ __forceinline__ int opaque(int v) {
     return v;
 }
 
-// LDS base registers of one candidate row: bases for the MRI planes (PPB
-// planes each) and for the CT planes, hidden from the optimiser so that each
-// stays one register and the plane, row and dy offsets ride in the ds_read
-// immediates.  `base` is the LDS element index of plane 0, row 0 of the
-// lane's window for dy = 0.
-template <class C>
-__device__ __forceinline__ void make_bases(int base, int (&ba)[C::NBA], int (&bl)[C::NBL]) {
+// All 2 HW + 1 dx candidates of one dy with squared differences.  `base` is
+// the LDS element index of plane 0, row 0 of the lane's window for dx = 0.
+// Candidate dx reads MRI rows dx .. dx + BX + 1, so consecutive candidates
+// share BX + 1 of their BX + 2 rows: the rows stay in registers and each
+// candidate after the first reads one new row (BZ + 2 reads instead of
+// (BZ + 2)(BX + 2)); CT rows are shared the same way.  The planes are
+// addressed from a few base registers hidden from the optimiser (`opaque`),
+// so each stays one register and the plane and row offsets ride in the
+// ds_read immediates.  Per candidate the lane squares its (BZ+2) x (BX+2)
+// differences, sums them pairwise over x and then z in registers and over y
+// across lanes (two DPP adds), and weighs with w = 2^-SSD' on MRI values
+// pre-scaled by sqrt(log2(e) / div).
+template <class C, int BX, int BZ, int HW>
+__device__ __forceinline__ void tiled_dx_sweep(const float* __restrict__ lds, int base,
+                                               const float (&tv)[BZ + 2][BX + 2],
+                                               float (&P)[BZ][BX], float (&S)[BZ][BX]) {
+    int ba[C::NBAS], bl[C::NBLS];
 #pragma unroll
-    for (int b = 0; b < C::NBA; ++b) ba[b] = opaque(base + b * C::PPB * C::PS);
+    for (int b = 0; b < C::NBAS; ++b) ba[b] = opaque(base + b * C::PPBS * C::PS);
 #pragma unroll
-    for (int b = 0; b < C::NBL; ++b) bl[b] = opaque(base + C::LOFF + b * C::PPB * C::PS);
-}
-
-// One (dx, dy) candidate step for the lane's BZ x BX block, dy = `dy`
-// elements past the bases (a compile-time constant where the dy loop is
-// unrolled).
-template <class C, int BX, int BZ>
-__device__ __forceinline__ void tiled_step_at(const float* __restrict__ lds,
-                                              const int (&ba)[C::NBA], const int (&bl)[C::NBL],
-                                              int dy, const float (&tv)[BZ + 2][BX + 2],
-                                              float (&P)[BZ][BX], float (&S)[BZ][BX]) {
-    float ex[BZ + 2][BX];
+    for (int b = 0; b < C::NBLS; ++b) bl[b] = opaque(base + C::LOFF + b * C::PPBS * C::PS);
+    constexpr int NR = BX + 2 + 2 * HW;
+    float rows[NR][BZ + 2];
+    auto load_row = [&](int r) {
 #pragma unroll
-    for (int pz = 0; pz < BZ + 2; ++pz) {
-        float d[BX + 2];
-        const int pp = ba[pz / C::PPB] + (pz % C::PPB) * C::PS + dy;
+        for (int pz = 0; pz < BZ + 2; ++pz)
+            rows[r][pz] = lds[ba[pz / C::PPBS] + (pz % C::PPBS) * C::PS + r * C::CW];
+    };
 #pragma unroll
-        for (int i = 0; i < BX + 2; ++i) {
-            const float df = tv[pz][i] - lds[pp + i * C::CW];
-            d[i] = df * df;
+    for (int r = 0; r < BX + 1; ++r) load_row(r);
+#pragma unroll
+    for (int dx = 0; dx <= 2 * HW; ++dx) {
+        load_row(dx + BX + 1);
+        float ex[BZ + 2][BX];
+#pragma unroll
+        for (int pz = 0; pz < BZ + 2; ++pz) {
+            float d[BX + 2];
+#pragma unroll
+            for (int i = 0; i < BX + 2; ++i) {
+                const float df = tv[pz][i] - rows[dx + i][pz];
+                d[i] = df * df;
+            }
+#pragma unroll
+            for (int i = 0; i < BX; i += 2) {
+                const float q = d[i + 1] + d[i + 2];
+                ex[pz][i] = d[i] + q;
+                ex[pz][i + 1] = q + d[i + 3];
+            }
         }
 #pragma unroll
-        for (int i = 0; i < BX; i += 2) {
-            const float q = d[i + 1] + d[i + 2];
-            ex[pz][i] = d[i] + q;
-            ex[pz][i + 1] = q + d[i + 3];
+        for (int j = 0; j < BZ; j += 2) {
+#pragma unroll
+            for (int i = 0; i < BX; ++i) {
+                const float q = ex[j + 1][i] + ex[j + 2][i];
+                const float e0 = ex[j][i] + q;
+                const float e1 = q + ex[j + 3][i];
+                const float s0 = lane_next(e0) + (lane_prev(e0) + e0);
+                const float s1 = lane_next(e1) + (lane_prev(e1) + e1);
+                const float w0 = __builtin_amdgcn_exp2f(-s0);
+                const float w1 = __builtin_amdgcn_exp2f(-s1);
+                const float l0 =
+                    lds[bl[j / C::PPBS] + (j % C::PPBS) * C::PS + (dx + i + 1) * C::CW];
+                const float l1 = lds[bl[(j + 1) / C::PPBS] + ((j + 1) % C::PPBS) * C::PS +
+                                     (dx + i + 1) * C::CW];
+                P[j][i] = fmaf(w0, l0, P[j][i]);
+                S[j][i] += w0;
+                P[j + 1][i] = fmaf(w1, l1, P[j + 1][i]);
+                S[j + 1][i] += w1;
+            }
         }
     }
-#pragma unroll
-    for (int j = 0; j < BZ; j += 2) {
-#pragma unroll
-        for (int i = 0; i < BX; ++i) {
-            const float q = ex[j + 1][i] + ex[j + 2][i];
-            const float e0 = ex[j][i] + q;
-            const float e1 = q + ex[j + 3][i];
-            const float s0 = lane_next(e0) + (lane_prev(e0) + e0);
-            const float s1 = lane_next(e1) + (lane_prev(e1) + e1);
-            const float w0 = __builtin_amdgcn_exp2f(-s0);
-            const float w1 = __builtin_amdgcn_exp2f(-s1);
-            const float l0 = lds[bl[j / C::PPB] + (j % C::PPB) * C::PS + (i + 1) * C::CW + dy];
-            const float l1 =
-                lds[bl[(j + 1) / C::PPB] + ((j + 1) % C::PPB) * C::PS + (i + 1) * C::CW + dy];
-            P[j][i] = fmaf(w0, l0, P[j][i]);
-            S[j][i] += w0;
-            P[j + 1][i] = fmaf(w1, l1, P[j + 1][i]);
-            S[j + 1][i] += w1;
-        }
-    }
-}
-
-// One (dx, dy) candidate step at LDS element index `base` (plane 0, row 0 of
-// the lane's window for this step).
-template <class C, int BX, int BZ>
-__device__ __forceinline__ void tiled_step(const float* __restrict__ lds, int base,
-                                           const float (&tv)[BZ + 2][BX + 2],
-                                           float (&P)[BZ][BX], float (&S)[BZ][BX]) {
-    int ba[C::NBA], bl[C::NBL];
-    make_bases<C>(base, ba, bl);
-    tiled_step_at<C, BX, BZ>(lds, ba, bl, 0, tv, P, S);
 }
 
 // Staging loads read clamped positions (always inside the volume) and
@@ -414,15 +422,10 @@ __device__ __forceinline__ void tiled_item(const TiledArgs& args, int item,
             stage_slab<C, NW>(lds, A, L, args.scale, z0 + dz, xs, ys, nx, ny, nz, plane, wv, lane);
             __syncthreads();
             if (!live) continue;
-            // (dx, dy) loops stay rolled: unrolling dy removes the per-step
-            // base adds but multiplies the code size, and the batched graph
-            // runs several of these kernels side by side
+            // dy rolled, dx unrolled: consecutive dx share MRI and CT rows
 #pragma unroll 1
-            for (int dx = 0; dx <= 2 * HW; ++dx) {
-#pragma unroll 1
-                for (int dy = 0; dy <= 2 * HW; ++dy)
-                    tiled_step<C, BX, BZ>(lds, lane_base + dx * C::CW + dy, tv, P, S);
-            }
+            for (int dy = 0; dy <= 2 * HW; ++dy)
+                tiled_dx_sweep<C, BX, BZ, HW>(lds, lane_base + dy, tv, P, S);
         }
     }
 
@@ -472,9 +475,10 @@ __global__ __launch_bounds__(NW * 64) void fuse_tiled(TiledArgs args) {
 //
 // Every LDS read is 64 bits wide, which costs one instruction slot like a
 // 32-bit read on gfx950: the MRI planes are interleaved in pairs, so a read
-// returns two planes at a position, and (L~, E) is one pair.  All reads of a
-// candidate address one base register (plane offsets ride in the 16-bit
-// immediates), one add per candidate.
+// returns two planes at a position, and (L~, E) is one pair.  The candidates
+// of one dy are swept over dx with the MRI rows kept in registers (one new
+// row per candidate), and all their reads address one base register per dy
+// (plane and row offsets ride in the 16-bit immediates).
 //
 // The expanded form loses absolute precision in proportion to TT + AA, and
 // 2^C must not overflow, so it is used only where every |T'| and |A'| is at
@@ -518,6 +522,11 @@ struct XtCfg {
     static constexpr int PPP = (8191 - ((C::BXv + 1) * C::CW + 2 * C::HWv)) / C::PS + 1;
     static_assert(PPP >= 1, "one pair plane must fit an LDS immediate window");
     static constexpr int NB = (NPP + PPP - 1) / PPP;
+    // the same for the dx sweep, whose immediates carry the row (up to
+    // 2 HW + BX + 1) and no dy
+    static constexpr int PPPS = (8191 - (2 * C::HWv + C::BXv + 1) * C::CW) / C::PS + 1;
+    static_assert(PPPS >= 1, "one pair plane must fit an LDS immediate window");
+    static constexpr int NBS = (NPP + PPPS - 1) / PPPS;
     // float index of MRI plane pz, and of the (L~, E) pair of CT plane j, at
     // in-plane position p
     static __device__ __forceinline__ int mri(int pz, int p) {
@@ -544,6 +553,24 @@ __device__ __forceinline__ f32x2 pair_at(const unsigned (&b)[XtCfg<C>::NB], int 
     return q[(pl % X::PPP) * C::PS + off];
 }
 
+// Bases and pair reads of the dx sweep (see XtCfg::PPPS).
+template <class C>
+__device__ __forceinline__ void make_bases_sweep(unsigned lds0, int pos,
+                                                 unsigned (&b)[XtCfg<C>::NBS]) {
+    using X = XtCfg<C>;
+#pragma unroll
+    for (int k = 0; k < X::NBS; ++k)
+        b[k] = opaque_u(lds0 + 8u * (unsigned)(pos + k * X::PPPS * C::PS));
+}
+
+template <class C>
+__device__ __forceinline__ f32x2 pair_at_sweep(const unsigned (&b)[XtCfg<C>::NBS], int pl,
+                                               int off) {
+    using X = XtCfg<C>;
+    const lds_pair_t* q = (const lds_pair_t*)(size_t)b[pl / X::PPPS];
+    return q[(pl % X::PPPS) * C::PS + off];
+}
+
 // The lane's MRI values of one candidate: a[pz][i] = plane pz, row i.
 template <class C, int BX, int BZ>
 __device__ __forceinline__ void load_mri_xt(const unsigned (&b)[XtCfg<C>::NB],
@@ -559,42 +586,63 @@ __device__ __forceinline__ void load_mri_xt(const unsigned (&b)[XtCfg<C>::NB],
         }
 }
 
-// One (dx, dy) candidate in cross-term form; t2 holds 2 T'.
-template <class C, int BX, int BZ>
-__device__ __forceinline__ void xterm_step(const unsigned (&b)[XtCfg<C>::NB],
-                                           const float (&t2)[BZ + 2][BX + 2],
-                                           float (&P)[BZ][BX], float (&S)[BZ][BX]) {
+// All 2 HW + 1 dx candidates of one dy in cross-term form.  Candidate dx
+// reads MRI rows dx .. dx + BX + 1, so consecutive candidates share BX + 1
+// of their BX + 2 rows: the rows stay in registers and each candidate after
+// the first reads one new row (MPAIRS 64-bit reads instead of
+// MPAIRS (BX + 2)).  Row and plane offsets ride in the immediates of the
+// dy's bases.  Per candidate the lane forms C with separable sums (x sums as
+// FMA chains of products, then z pairwise, then y across lanes), then
+// w~ = 2^C, P += w~ L~, S += w~ E; t2 holds 2 T'.
+template <class C, int BX, int BZ, int HW>
+__device__ __forceinline__ void xterm_dx_sweep(const unsigned (&b)[XtCfg<C>::NBS],
+                                               const float (&t2)[BZ + 2][BX + 2],
+                                               float (&P)[BZ][BX], float (&S)[BZ][BX]) {
     using X = XtCfg<C>;
-    float a[BZ + 2][BX + 2];
-    load_mri_xt<C, BX, BZ>(b, a);
-    float ex[BZ + 2][BX];
+    constexpr int NR = BX + 2 + 2 * HW;
+    float rows[NR][BZ + 2];
+    auto load_row = [&](int r) {
 #pragma unroll
-    for (int pz = 0; pz < BZ + 2; ++pz) {
-#pragma unroll
-        for (int i = 0; i < BX; i += 2) {
-            float q = t2[pz][i + 1] * a[pz][i + 1];
-            q = fmaf(t2[pz][i + 2], a[pz][i + 2], q);
-            ex[pz][i] = fmaf(t2[pz][i], a[pz][i], q);
-            ex[pz][i + 1] = fmaf(t2[pz][i + 3], a[pz][i + 3], q);
+        for (int k = 0; k < X::MPAIRS; ++k) {
+            const f32x2 v = pair_at_sweep<C>(b, k, r * C::CW);
+            rows[r][2 * k] = v.x;
+            rows[r][2 * k + 1] = v.y;
         }
-    }
+    };
 #pragma unroll
-    for (int j = 0; j < BZ; j += 2) {
+    for (int r = 0; r < BX + 1; ++r) load_row(r);
 #pragma unroll
-        for (int i = 0; i < BX; ++i) {
-            const float q = ex[j + 1][i] + ex[j + 2][i];
-            const float e0 = ex[j][i] + q;
-            const float e1 = q + ex[j + 3][i];
-            const float c0 = lane_next(e0) + (lane_prev(e0) + e0);
-            const float c1 = lane_next(e1) + (lane_prev(e1) + e1);
-            const float w0 = __builtin_amdgcn_exp2f(c0);
-            const float w1 = __builtin_amdgcn_exp2f(c1);
-            const f32x2 g0 = pair_at<C>(b, X::MPAIRS + j, (i + 1) * C::CW);
-            const f32x2 g1 = pair_at<C>(b, X::MPAIRS + j + 1, (i + 1) * C::CW);
-            P[j][i] = fmaf(w0, g0.x, P[j][i]);
-            S[j][i] = fmaf(w0, g0.y, S[j][i]);
-            P[j + 1][i] = fmaf(w1, g1.x, P[j + 1][i]);
-            S[j + 1][i] = fmaf(w1, g1.y, S[j + 1][i]);
+    for (int dx = 0; dx <= 2 * HW; ++dx) {
+        load_row(dx + BX + 1);
+        float ex[BZ + 2][BX];
+#pragma unroll
+        for (int pz = 0; pz < BZ + 2; ++pz) {
+#pragma unroll
+            for (int i = 0; i < BX; i += 2) {
+                float q = t2[pz][i + 1] * rows[dx + i + 1][pz];
+                q = fmaf(t2[pz][i + 2], rows[dx + i + 2][pz], q);
+                ex[pz][i] = fmaf(t2[pz][i], rows[dx + i][pz], q);
+                ex[pz][i + 1] = fmaf(t2[pz][i + 3], rows[dx + i + 3][pz], q);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < BZ; j += 2) {
+#pragma unroll
+            for (int i = 0; i < BX; ++i) {
+                const float q = ex[j + 1][i] + ex[j + 2][i];
+                const float e0 = ex[j][i] + q;
+                const float e1 = q + ex[j + 3][i];
+                const float c0 = lane_next(e0) + (lane_prev(e0) + e0);
+                const float c1 = lane_next(e1) + (lane_prev(e1) + e1);
+                const float w0 = __builtin_amdgcn_exp2f(c0);
+                const float w1 = __builtin_amdgcn_exp2f(c1);
+                const f32x2 g0 = pair_at_sweep<C>(b, X::MPAIRS + j, (dx + i + 1) * C::CW);
+                const f32x2 g1 = pair_at_sweep<C>(b, X::MPAIRS + j + 1, (dx + i + 1) * C::CW);
+                P[j][i] = fmaf(w0, g0.x, P[j][i]);
+                S[j][i] = fmaf(w0, g0.y, S[j][i]);
+                P[j + 1][i] = fmaf(w1, g1.x, P[j + 1][i]);
+                S[j + 1][i] = fmaf(w1, g1.y, S[j + 1][i]);
+            }
         }
     }
 }
@@ -846,11 +894,8 @@ __device__ __forceinline__ void tiled_item_xt(const TiledArgs& args, int item,
                 __syncthreads();
                 if (!live) continue;
 #pragma unroll 1
-                for (int dx = 0; dx <= 2 * HW; ++dx) {
-#pragma unroll 1
-                    for (int dy = 0; dy <= 2 * HW; ++dy)
-                        tiled_step<C, BX, BZ>(lds, lane_base + dx * C::CW + dy, tv, P, S);
-                }
+                for (int dy = 0; dy <= 2 * HW; ++dy)
+                    tiled_dx_sweep<C, BX, BZ, HW>(lds, lane_base + dy, tv, P, S);
             }
         }
 #pragma unroll
@@ -887,16 +932,10 @@ __device__ __forceinline__ void tiled_item_xt(const TiledArgs& args, int item,
                 if (__syncthreads_or(flag) == 0) {
                     if (!live) continue;
 #pragma unroll 1
-                    for (int dx = 0; dx <= 2 * HW; ++dx) {
-                        unsigned b[X::NB];
-                        make_bases_xt<C>(lds0, lane_base + dx * C::CW, b);
-#pragma unroll 1
-                        for (int dy = 0; dy <= 2 * HW; ++dy) {
-                            unsigned bb[X::NB];
-#pragma unroll
-                            for (int k = 0; k < X::NB; ++k) bb[k] = b[k] + 8u * (unsigned)dy;
-                            xterm_step<C, BX, BZ>(bb, tv, P, S);
-                        }
+                    for (int dy = 0; dy <= 2 * HW; ++dy) {
+                        unsigned b[X::NBS];
+                        make_bases_sweep<C>(lds0, lane_base + dy, b);
+                        xterm_dx_sweep<C, BX, BZ, HW>(b, tv, P, S);
                     }
                 } else {
                     restage_ct_raw<C, NW>(lds, L, z0 + dz, xs, ys, nx, ny, nz, plane, wv, lane);
