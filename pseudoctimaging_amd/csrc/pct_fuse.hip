@@ -422,13 +422,16 @@ static double model_cost(const pct_problem* pb, int shape_id, int group, TileGeo
 // while the best shapes of a window lie within 5 % of each other.
 int preferred_shape(int hw) {
     switch (hw) {
-        // the cross-term {32, 2, 2, 4} (16) against the fastest squared-difference
-        // shape of each window: 0.37 vs 0.44 ms (160x160x96, n=1), 1.07 vs 1.17
-        // (128x128x64, n=10); 2.67 vs 2.72 (192x192x96, n=3), 19.1 vs 21.2
-        // (256x256x128, n=10); 10.0 vs 11.6 (222x222x112, n=5), 19.6 vs 22.3
-        // (n=10)
+        // every shape timed at group 1 and 2 on one box (ms): NH 7: the
+        // cross-term {32, 2, 2, 4} (16) 0.318 against 0.312 for {64, 2, 2, 4}
+        // (19) at 160x160x96, n=1, and 0.928 against 0.946 at 128x128x64,
+        // n=10; NH 9: 19 2.124 against 2.366 for 16 at 192x192x96, n=3, and
+        // 16.65 against 17.03 at 256x256x128, n=10; NH 11: 16 8.93 and
+        // 17.13 at 222x222x112, n=5 and 10, against 9.03 and 17.79 for
+        // {32, 2, 2, 8} (17) and 9.85 and 19.62 for the best
+        // squared-difference shape
         case 3: return 16;
-        case 4: return 16;
+        case 4: return 19;
         case 5: return 16;
         default: return -1;
     }
