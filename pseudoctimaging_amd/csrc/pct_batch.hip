@@ -85,7 +85,7 @@ __global__ __launch_bounds__(NW * 64) void fuse_batched(BatchArgs args) {
     }
 }
 
-// fuse_batched with the cross-term items (TiledArgs::aa filled per subject).
+// fuse_batched with the cross-term items (TiledArgs::le filled per subject).
 // The item loop costs the compiler some 18 VGPRs over the one-item kernel;
 // past 128 a SIMD holds three waves instead of four, which the LDS of the
 // narrow shapes (37.6 KB at W 32, BX 2, BZ 2, HW 5) would allow, and the
@@ -282,17 +282,17 @@ static int enqueue_batch(const pct_subject* s, const BatchPlan& p, const std::ve
     const bool concurrent = !p.choice.serial && nc > 1;
     hipLaunchKernelGGL(reset_queues, dim3(1), dim3(64), 0, stream, counters, nc * kChunks);
     if (hipGetLastError() != hipSuccess) return PCT_ERR_HIP;
-    // AA of the atlases of every subject of a cross-term class, before the
-    // class kernels (and the fork of concurrent classes)
+    // cross-term pairs of the atlases of every subject of a cross-term class,
+    // before the class kernels (and the fork of concurrent classes)
     for (int c = 0; c < nc; ++c) {
         if (!is_xterm_shape(p.choice.shape_id[p.class_hw[c]])) continue;
         for (int i : p.classes[c]) {
             const TiledArgs t = make_tiled_args(&s[i].pb, p.geo[i], s[i].d_img, s[i].d_atlas,
                                                 s[i].d_atlas_label_norm, nullptr);
-            const int rc = launch_atlas_aa(
-                &s[i].pb, s[i].d_atlas, s[i].pb.n_atlas, t.scale,
-                reinterpret_cast<float*>(base + p.ws_off[i] +
-                                         tiled_aa_offset(&s[i].pb, p.geo[i].n_groups)),
+            const int rc = launch_atlas_pairs(
+                &s[i].pb, s[i].d_atlas, s[i].d_atlas_label_norm, s[i].pb.n_atlas, t.scale,
+                reinterpret_cast<f32x2*>(base + p.ws_off[i] +
+                                         tiled_le_offset(&s[i].pb, p.geo[i].n_groups)),
                 stream);
             if (rc) return rc;
         }
@@ -315,8 +315,8 @@ static int enqueue_batch(const pct_subject* s, const BatchPlan& p, const std::ve
                                        s[i].d_atlas_label_norm,
                                        reinterpret_cast<float*>(base + p.ws_off[i]));
             if (is_xterm_shape(sid))
-                a.sub[k].aa = reinterpret_cast<const float*>(
-                    base + p.ws_off[i] + tiled_aa_offset(&s[i].pb, p.geo[i].n_groups));
+                a.sub[k].le = reinterpret_cast<const f32x2*>(
+                    base + p.ws_off[i] + tiled_le_offset(&s[i].pb, p.geo[i].n_groups));
             a.n_items[k] = p.geo[i].n_items;
             ++k;
         }

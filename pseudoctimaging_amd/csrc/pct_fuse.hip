@@ -112,25 +112,29 @@ __global__ __launch_bounds__(256) void fuse_direct(const float* __restrict__ img
     out[idx] = fused_value(P, S);
 }
 
-// AA of every voxel of atlases [0, n_atlas): the 3x3x3 sum of (scale * A)^2,
-// zero outside the volume.  A workgroup owns 64 y x 4 x columns of one chunk
-// of zc planes and sweeps it in z: the in-plane 3x3 sums of each plane's
-// (4+2) x (64+2) tile, staged through LDS, enter a running sum over three
-// planes.  Four planes are loaded per step (eight loads in flight per lane
-// instead of one, issued before the previous step's sums), and chunking z (two halo planes per chunk) gives the grid
-// enough workgroups to hide HBM latency on a single subject.  Every output
-// is ((s(z-1) + s(z)) + s(z+1)) with each in-plane sum in (dx, dy) order,
-// whatever the chunking.
+// The cross-term pairs of every voxel of atlases [0, n_atlas): with AA the
+// 3x3x3 sum of (scale * A)^2 (zero outside the volume) and E = 2^-AA, the
+// pair (L * E, E), or (+inf, E) where the CT value L lies beyond
+// kXtLabelBound or is not a number, which the slab staging flags.  A
+// workgroup owns 64 y x 4 x columns of one chunk of zc planes and sweeps it
+// in z: the in-plane 3x3 sums of each plane's (4+2) x (64+2) tile, staged
+// through LDS, enter a running sum over three planes.  Four planes are
+// loaded per step (eight loads in flight per lane instead of one, issued
+// with the step's CT loads before the previous step's sums), and chunking z
+// (two halo planes per chunk) gives the grid enough workgroups to hide HBM
+// latency on a single subject.  Every AA is ((s(z-1) + s(z)) + s(z+1)) with
+// each in-plane sum in (dx, dy) order, whatever the chunking.
 constexpr int kSqBoxY = 64, kSqBoxX = 4, kSqStep = 4;
 constexpr int kSqThreads = kSqBoxY * kSqBoxX;
 constexpr int kSqTile = (kSqBoxX + 2) * (kSqBoxY + 2);
 constexpr int kSqLoads = (kSqTile + kSqThreads - 1) / kSqThreads;
 constexpr int kSqMinChunk = 8;
 __global__ __launch_bounds__(kSqThreads) void atlas_sq_box(const float* __restrict__ atlas,
+                                                           const float* __restrict__ label,
                                                            long long N, int nx, int ny, int nz,
                                                            int n_atlas, int zc, int n_chunks,
                                                            float scale,
-                                                           float* __restrict__ aa) {
+                                                           f32x2* __restrict__ le) {
     __shared__ float tile[kSqStep * kSqTile];
     const int ly = threadIdx.x % kSqBoxY, lx = threadIdx.x / kSqBoxY;
     const int y0 = blockIdx.x * kSqBoxY, x0 = blockIdx.y * kSqBoxX;
@@ -152,7 +156,10 @@ __global__ __launch_bounds__(kSqThreads) void atlas_sq_box(const float* __restri
         const int at = wi / n_chunks, ck = wi - at * n_chunks;
         const int zs = ck * zc, ze = min(nz, zs + zc);   // output planes [zs, ze)
         const float* __restrict__ A = atlas + (long long)at * N;
-        float* __restrict__ out = aa + (long long)at * N;
+        const float* __restrict__ L = label + (long long)at * N;
+        f32x2* __restrict__ out = le + (long long)at * N;
+        const bool col_in = x < nx && y < ny;
+        const long long col = col_in ? (long long)x * ny + y : 0;
         float s_prev = 0.f, s_cur = 0.f;   // in-plane sums of planes q-2 and q-1
         // planes p .. p+3 of the lane's tile elements, scaled; zero outside
         // the volume and past the chunk's last plane ze
@@ -167,7 +174,18 @@ __global__ __launch_bounds__(kSqThreads) void atlas_sq_box(const float* __restri
                     v[k][j] = (qin && in[j]) ? A[q * plane + off[j]] * scale : 0.f;
             }
         };
+        // CT values of output planes p-1 .. p+2 (plane q completes output q-1)
+        float lab[kSqStep];
+        auto load_labels = [&](int p) {
+#pragma unroll
+            for (int k = 0; k < kSqStep; ++k) {
+                const int q = p + k;
+                const float raw = L[(long long)clampi(q - 1, nz) * plane + col];
+                lab[k] = (q - 1 >= zs && q <= ze && col_in) ? raw : 0.f;
+            }
+        };
         load_step(zs - 1);
+        load_labels(zs - 1);
         // planes zs-1 .. ze: plane q completes output plane q-1
         for (int p = zs - 1; p <= ze; p += kSqStep) {
             __syncthreads();   // the previous step's reads of the tiles are done
@@ -179,8 +197,14 @@ __global__ __launch_bounds__(kSqThreads) void atlas_sq_box(const float* __restri
                     if (e < kSqTile) tile[k * kSqTile + e] = v[k][j];
                 }
             __syncthreads();
+            float lab_now[kSqStep];
+#pragma unroll
+            for (int k = 0; k < kSqStep; ++k) lab_now[k] = lab[k];
             // the next step's loads are in flight during this step's sums
-            if (p + kSqStep <= ze) load_step(p + kSqStep);
+            if (p + kSqStep <= ze) {
+                load_step(p + kSqStep);
+                load_labels(p + kSqStep);
+            }
 #pragma unroll
             for (int k = 0; k < kSqStep; ++k) {
                 const int q = p + k;
@@ -194,8 +218,12 @@ __global__ __launch_bounds__(kSqThreads) void atlas_sq_box(const float* __restri
                         const float w = t[(lx + dx) * (kSqBoxY + 2) + ly + dy];
                         s = fmaf(w, w, s);
                     }
-                if (q - 1 >= zs && x < nx && y < ny)
-                    out[(q - 1) * plane + (long long)x * ny + y] = (s_prev + s_cur) + s;
+                if (q - 1 >= zs && col_in) {
+                    const float e = __builtin_amdgcn_exp2f(-((s_prev + s_cur) + s));
+                    const float l = lab_now[k];
+                    out[(q - 1) * plane + col] =
+                        f32x2{fabsf(l) <= kXtLabelBound ? l * e : __builtin_inff(), e};
+                }
                 s_prev = s_cur;
                 s_cur = s;
             }
@@ -560,21 +588,21 @@ bool is_xterm_shape(int shape_id) {
     return shape_id >= 0 && shape_id < (int)shapes().size() && shapes()[shape_id].form == 1;
 }
 
-size_t tiled_aa_offset(const pct_problem* pb, int n_groups) {
+size_t tiled_le_offset(const pct_problem* pb, int n_groups) {
     const size_t N = (size_t)pb->nx * pb->ny * pb->nz;
     return ((2 * (size_t)n_groups * N * sizeof(float)) + 255) & ~(size_t)255;
 }
 
 size_t tiled_ws_bytes(const pct_problem* pb, int n_groups, int shape_id) {
     const size_t N = (size_t)pb->nx * pb->ny * pb->nz;
-    size_t bytes = tiled_aa_offset(pb, n_groups);
+    size_t bytes = tiled_le_offset(pb, n_groups);
     if (is_xterm_shape(shape_id))
-        bytes += ((size_t)pb->n_atlas * N * sizeof(float) + 255) & ~(size_t)255;
+        bytes += ((size_t)pb->n_atlas * N * sizeof(f32x2) + 255) & ~(size_t)255;
     return bytes;
 }
 
-int launch_atlas_aa(const pct_problem* pb, const float* atlas, int n_atlas, float scale,
-                    float* aa, hipStream_t stream) {
+int launch_atlas_pairs(const pct_problem* pb, const float* atlas, const float* label,
+                       int n_atlas, float scale, f32x2* le, hipStream_t stream) {
     if (n_atlas <= 0) return PCT_OK;
     const long long N = (long long)pb->nx * pb->ny * pb->nz;
     const int gy = (pb->ny + kSqBoxY - 1) / kSqBoxY, gx = (pb->nx + kSqBoxX - 1) / kSqBoxX;
@@ -587,8 +615,8 @@ int launch_atlas_aa(const pct_problem* pb, const float* atlas, int n_atlas, floa
     const int zc = (pb->nz + n_chunks - 1) / n_chunks;
     n_chunks = (pb->nz + zc - 1) / zc;
     const dim3 grid(gy, gx, (unsigned)std::min<long long>((long long)n_atlas * n_chunks, 65535));
-    hipLaunchKernelGGL(atlas_sq_box, grid, dim3(kSqThreads), 0, stream, atlas, N, pb->nx, pb->ny,
-                       pb->nz, n_atlas, zc, n_chunks, scale, aa);
+    hipLaunchKernelGGL(atlas_sq_box, grid, dim3(kSqThreads), 0, stream, atlas, label, N, pb->nx,
+                       pb->ny, pb->nz, n_atlas, zc, n_chunks, scale, le);
     return hipGetLastError() == hipSuccess ? PCT_OK : PCT_ERR_HIP;
 }
 
@@ -634,7 +662,7 @@ TiledArgs make_tiled_args(const pct_problem* pb, const TileGeometry& g, const fl
     const double div = problem_divisor(pb);
     a.scale = (float)std::sqrt(1.4426950408889634 / div);
     a.only = nullptr;
-    a.aa = nullptr;
+    a.le = nullptr;
     return a;
 }
 
@@ -643,7 +671,7 @@ TiledArgs make_tiled_args(const pct_problem* pb, const TileGeometry& g, const fl
 int launch_tiled_items(const TiledArgs& a, int shape_id, int hw, hipStream_t stream) {
     const TiledEntry* e = find_entry(shape_id, hw);
     if (!e) return PCT_ERR_UNSUPPORTED;
-    if (is_xterm_shape(shape_id) && !a.aa) return PCT_ERR_INVALID;   // AA pass not run
+    if (is_xterm_shape(shape_id) && !a.le) return PCT_ERR_INVALID;   // pair pass not run
     if (a.n_items <= 0) return PCT_OK;
     TiledArgs copy = a;
     void* kargs[] = {&copy};
@@ -697,11 +725,11 @@ static int launch_tiled(const pct_problem* pb, const float* img, const float* at
     if (ws_bytes < tiled_ws_bytes(pb, g.n_groups, shape_id)) return PCT_ERR_INVALID;
     TiledArgs a = make_tiled_args(pb, g, img, atlas, label, static_cast<float*>(ws));
     if (is_xterm_shape(shape_id) && g.n_items > 0) {
-        float* aa = reinterpret_cast<float*>(static_cast<char*>(ws) +
-                                             tiled_aa_offset(pb, g.n_groups));
-        const int rc = launch_atlas_aa(pb, atlas, pb->n_atlas, a.scale, aa, stream);
+        f32x2* le = reinterpret_cast<f32x2*>(static_cast<char*>(ws) +
+                                             tiled_le_offset(pb, g.n_groups));
+        const int rc = launch_atlas_pairs(pb, atlas, label, pb->n_atlas, a.scale, le, stream);
         if (rc) return rc;
-        a.aa = aa;
+        a.le = le;
     }
     if (g.n_items > 0) {
         void* kargs[] = {&a};
@@ -845,7 +873,7 @@ extern "C" int pct_fuse_dev_ws(const pct_problem* pb, const float* d_img, const 
 // run uses one atlas group (every item sums all atlases, so the partials are
 // 2 N floats and every interior voxel is written by exactly one item before
 // the finalize pass reads it: nothing needs clearing) and the
-// squared-difference kernels, which need no AA workspace.
+// squared-difference kernels, which need no cross-term pairs.
 
 constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
@@ -935,7 +963,7 @@ struct HostCallState {
     std::mutex mu;
     float *img = nullptr, *atlas = nullptr, *label = nullptr, *out = nullptr;
     size_t n_cap = 0, na_cap = 0;
-    // workspace of the host-buffer calls (partial sums, AA, post-chain
+    // workspace of the host-buffer calls (partial sums, cross-term pairs, post-chain
     // lists): their own, so that they never share memory with device-buffer
     // calls still running on the caller's streams
     void* ws = nullptr;
@@ -1132,9 +1160,9 @@ static int fuse_host_impl(const pct_problem* pb, const float* img, const float* 
         if (!need) need = 256;
         rc = host_workspace(st, need, &ws);
     }
-    float* aa = tiled && is_xterm_shape(sid)
-                    ? reinterpret_cast<float*>(static_cast<char*>(ws) +
-                                               tiled_aa_offset(pb, g.n_groups))
+    f32x2* le = tiled && is_xterm_shape(sid)
+                    ? reinterpret_cast<f32x2*>(static_cast<char*>(ws) +
+                                               tiled_le_offset(pb, g.n_groups))
                     : nullptr;
     // cu:1687-1699: host->device copies
     if (!rc) rc = stage_h2d(st, st.img, img, N * 4, cs);
@@ -1159,10 +1187,11 @@ static int fuse_host_impl(const pct_problem* pb, const float* img, const float* 
             a.n_atlas = a1 - a0;
             a.n_items = (int)(g.n_tiles * (g1 - g0));
             hipStream_t cstream = (c & 1) ? st.compute2 : s;
-            if (aa) {
-                // AA of the chunk's atlases, computed as the single launch does
-                rc = launch_atlas_aa(pb, st.atlas + off, a1 - a0, a.scale, aa + off, cstream);
-                a.aa = aa + off;
+            if (le) {
+                // pairs of the chunk's atlases, computed as the single launch does
+                rc = launch_atlas_pairs(pb, st.atlas + off, st.label + off, a1 - a0, a.scale,
+                                        le + off, cstream);
+                a.le = le + off;
             }
             if (!rc) rc = launch_tiled_items(a, sid, g.hw, cstream);
         }

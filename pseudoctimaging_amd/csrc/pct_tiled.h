@@ -44,6 +44,10 @@ __device__ __forceinline__ float fused_value(float P, float S) {
 // tiled kernel
 // ---------------------------------------------------------------------------
 
+// Native two-float vector: its loads and stores stay single 64-bit accesses
+// (HIP's float2 struct is copied member by member).
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
 struct TiledArgs {
     const float* img;
     const float* atlas;
@@ -59,9 +63,10 @@ struct TiledArgs {
     float scale;      // sqrt(log2(e)/div)
     // optional item filter: when set, items whose byte is 0 are skipped
     const unsigned char* only;
-    // cross-term shapes only: AA of every atlas voxel ([n_atlas][N]), the
-    // 3x3x3 sum of the squared scaled atlas MRI (atlas_sq_box)
-    const float* aa;
+    // cross-term shapes only: per atlas voxel the pair (L * E, E) with
+    // E = 2^-AA and AA the 3x3x3 sum of the squared scaled atlas MRI
+    // ([n_atlas][N] pairs, written by atlas_sq_box)
+    const f32x2* le;
 };
 
 template <int W, int BX, int BZ, int NW, int HW>
@@ -491,9 +496,6 @@ __global__ __launch_bounds__(NW * 64) void fuse_tiled(TiledArgs args) {
 constexpr float kXtBound = 0.25f;
 constexpr float kXtLabelBound = 1.8446744e19f;   // 2^64: bounds P / 2^-TT
 
-// Native two-float vector: its loads and stores stay single 64-bit LDS
-// accesses (HIP's float2 struct is copied member by member).
-typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) const f32x2 lds_pair_t;
 
 // LDS byte address of an element of the dynamic shared array.
@@ -698,18 +700,21 @@ __device__ __forceinline__ void scaled_ssd_step(const unsigned (&b)[XtCfg<C>::NB
 
 // Copies the slab of one (atlas, dz) stage in the cross-term layout: MRI
 // planes z = zdz-1 .. zdz+BZ (scaled, interleaved in pairs) and, for CT
-// planes z = zdz .. zdz+BZ-1, (L * E, E) with E = 2^-AA; positions outside
-// the volume hold zeros.  Returns the lane's flag: a staged MRI value beyond
-// kXtBound or CT value beyond kXtLabelBound (or either not a number).
+// planes z = zdz .. zdz+BZ-1, the pairs (L * E, E) of the AA pre-pass;
+// positions outside the volume hold zeros.  Returns whether any lane of the
+// wave staged an MRI value beyond kXtBound (or not a number) or a CT value
+// beyond kXtLabelBound (the pre-pass writes +inf in place of L * E for a CT
+// value beyond the bound or not a number, and |L * E| <= |L| otherwise).
+// The checks are wave masks: one compare per value, the OR on the scalar
+// unit.
 template <class C, int NW>
 __device__ __forceinline__ int stage_slab_xt(float* __restrict__ lds, const float* __restrict__ A,
-                                             const float* __restrict__ L,
-                                             const float* __restrict__ AA, float scale, int zdz,
+                                             const f32x2* __restrict__ LE, float scale, int zdz,
                                              int xs, int ys, int nx, int ny, int nz,
                                              long long plane, int wv, int lane) {
     using X = XtCfg<C>;
     constexpr int RPW = (C::RX + NW - 1) / NW;   // rows per wave and plane
-    int out_of_range = 0;
+    unsigned long long bad = 0;
     f32x2* __restrict__ lp = reinterpret_cast<f32x2*>(lds);
     int ycl[C::COLCHUNKS];
     bool yin[C::COLCHUNKS];
@@ -733,7 +738,7 @@ __device__ __forceinline__ int stage_slab_xt(float* __restrict__ lds, const floa
                 for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
                     const float raw = Ar[ycl[cc]];
                     const float t = (xok && yin[cc]) ? raw * scale : 0.f;
-                    out_of_range |= !(fabsf(t) <= kXtBound);
+                    bad |= __ballot(!(fabsf(t) <= kXtBound));
                     v[r][cc][h] = t;
                 }
             }
@@ -748,27 +753,35 @@ __device__ __forceinline__ int stage_slab_xt(float* __restrict__ lds, const floa
             }
         }
     }
-    // CT planes with their E
+    // CT planes: the (L * E, E) pairs
 #pragma unroll 1
     for (int j = 0; j < C::LPLANES; ++j) {
         const int zz = zdz + j;
         const bool zok = zz >= 0 && zz < nz;
         const long long zoff = (long long)clampi(zz, nz) * plane;
-        float lv[RPW][C::COLCHUNKS], av[RPW][C::COLCHUNKS];
+        f32x2 g[RPW][C::COLCHUNKS];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int xx = xs + wv + r * NW;
+            const long long ro = zoff + (long long)clampi(xx, nx) * ny;
+#pragma unroll
+            for (int cc = 0; cc < C::COLCHUNKS; ++cc) g[r][cc] = LE[ro + ycl[cc]];
+        }
+        // every load is issued before the first select: without the barrier
+        // the compiler moves each 64-bit load under its own branch and waits
+        // for it there
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int cc = 0; cc < C::COLCHUNKS; ++cc) asm volatile("" : "+v"(g[r][cc]));
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
             const int rr = wv + r * NW;
             const int xx = xs + rr;
             const bool xok = zok && rr < C::RX && xx >= 0 && xx < nx;
-            const long long ro = zoff + (long long)clampi(xx, nx) * ny;
 #pragma unroll
-            for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
-                const bool in = xok && yin[cc];
-                const float lraw = L[ro + ycl[cc]];
-                const float araw = AA[ro + ycl[cc]];
-                lv[r][cc] = in ? lraw : 0.f;
-                av[r][cc] = in ? araw : __builtin_inff();
-            }
+            for (int cc = 0; cc < C::COLCHUNKS; ++cc)
+                g[r][cc] = (xok && yin[cc]) ? g[r][cc] : f32x2{0.f, 0.f};
         }
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
@@ -776,14 +789,12 @@ __device__ __forceinline__ int stage_slab_xt(float* __restrict__ lds, const floa
 #pragma unroll
             for (int cc = 0; cc < C::COLCHUNKS; ++cc) {
                 const int c = lane + cc * 64;
-                out_of_range |= !(fabsf(lv[r][cc]) <= kXtLabelBound);
-                const float e = __builtin_amdgcn_exp2f(-av[r][cc]);   // 0 outside the volume
-                if (rr < C::RX && c < C::CW)
-                    lp[(X::MPAIRS + j) * C::PS + rr * C::CW + c] = f32x2{lv[r][cc] * e, e};
+                bad |= __ballot(fabsf(g[r][cc].x) > kXtLabelBound);
+                if (rr < C::RX && c < C::CW) lp[(X::MPAIRS + j) * C::PS + rr * C::CW + c] = g[r][cc];
             }
         }
     }
-    return out_of_range;
+    return bad != 0;
 }
 
 // Rewrites the first float of every (L~, E) pair of the slab with the raw CT
@@ -924,11 +935,11 @@ __device__ __forceinline__ void tiled_item_xt(const TiledArgs& args, int item,
         for (int a = a_begin; a < a_end; ++a) {
             const float* __restrict__ A = args.atlas + (long long)a * args.N;
             const float* __restrict__ L = args.label + (long long)a * args.N;
-            const float* __restrict__ AA = args.aa + (long long)a * args.N;
+            const f32x2* __restrict__ LE = args.le + (long long)a * args.N;
             for (int dz = -HW; dz <= HW; ++dz) {
                 __syncthreads();
-                const int flag = stage_slab_xt<C, NW>(lds, A, L, AA, args.scale, z0 + dz, xs, ys,
-                                                      nx, ny, nz, plane, wv, lane);
+                const int flag = stage_slab_xt<C, NW>(lds, A, LE, args.scale, z0 + dz, xs, ys, nx,
+                                                      ny, nz, plane, wv, lane);
                 if (__syncthreads_or(flag) == 0) {
                     if (!live) continue;
 #pragma unroll 1
@@ -1013,22 +1024,23 @@ int check_device(int* dev_out);
 int ensure_workspace(int dev, size_t bytes, void** ptr);
 
 // Bytes of workspace a tiled run of `shape_id` with `n_groups` atlas groups
-// needs: the partial sums, then for a cross-term shape the AA of every atlas
-// voxel (at tiled_aa_offset).
+// needs: the partial sums, then for a cross-term shape the (L * 2^-AA, 2^-AA)
+// pair of every atlas voxel (at tiled_le_offset).
 size_t tiled_ws_bytes(const pct_problem* pb, int n_groups, int shape_id);
-size_t tiled_aa_offset(const pct_problem* pb, int n_groups);
+size_t tiled_le_offset(const pct_problem* pb, int n_groups);
 
 // Bytes of workspace the device post chain needs for a volume, and the chain
 // itself on a caller's workspace of at least that size (pct_post.hip).
 size_t postprocess_ws_bytes(int nh, int nx, int ny, int nz);
 int postprocess_dev_ws(int nh, int nx, int ny, int nz, float* d_arr, void* ws, hipStream_t s);
 
-// Whether `shape_id` is a cross-term shape (its kernel reads TiledArgs::aa).
+// Whether `shape_id` is a cross-term shape (its kernel reads TiledArgs::le).
 bool is_xterm_shape(int shape_id);
 
-// Enqueues atlas_sq_box for atlases [0, n_atlas) of `atlas` into `aa`.
-int launch_atlas_aa(const pct_problem* pb, const float* atlas, int n_atlas, float scale,
-                    float* aa, hipStream_t stream);
+// Enqueues atlas_sq_box for atlases [0, n_atlas) of `atlas` and their CT
+// `label`: the (L * 2^-AA, 2^-AA) pairs into `le`.
+int launch_atlas_pairs(const pct_problem* pb, const float* atlas, const float* label,
+                       int n_atlas, float scale, f32x2* le, hipStream_t stream);
 
 TiledArgs make_tiled_args(const pct_problem* pb, const TileGeometry& g, const float* img,
                           const float* atlas, const float* label, float* ws);
