@@ -119,8 +119,8 @@ __global__ __launch_bounds__(256) void fuse_direct(const float* __restrict__ img
 // workgroup owns 64 y x 4 x columns of one chunk of zc planes and sweeps it
 // in z: the in-plane 3x3 sums of each plane's (4+2) x (64+2) tile, staged
 // through LDS, enter a running sum over three planes.  Four planes are
-// loaded per step (eight loads in flight per lane instead of one, issued
-// with the step's CT loads before the previous step's sums), and chunking z
+// loaded per step (eight loads in flight per lane, issued with the step's
+// CT loads before the previous step's sums), and chunking z
 // (two halo planes per chunk) gives the grid enough workgroups to hide HBM
 // latency on a single subject.  Every AA is ((s(z-1) + s(z)) + s(z+1)) with
 // each in-plane sum in (dx, dy) order, whatever the chunking.
@@ -161,41 +161,49 @@ __global__ __launch_bounds__(kSqThreads) void atlas_sq_box(const float* __restri
         const bool col_in = x < nx && y < ny;
         const long long col = col_in ? (long long)x * ny + y : 0;
         float s_prev = 0.f, s_cur = 0.f;   // in-plane sums of planes q-2 and q-1
-        // planes p .. p+3 of the lane's tile elements, scaled; zero outside
-        // the volume and past the chunk's last plane ze
+        // raw values of planes p .. p+3 at the lane's tile elements, loaded
+        // from clamped planes (always inside the volume) so that all of a
+        // step's loads are in flight together; the step that stores them
+        // zeroes the positions outside the volume and past the chunk's last
+        // plane ze, and scales the rest
         float v[kSqStep][kSqLoads];
         auto load_step = [&](int p) {
 #pragma unroll
             for (int k = 0; k < kSqStep; ++k) {
-                const int q = p + k;
-                const bool qin = q >= 0 && q < nz && q <= ze;
+                const long long qo = (long long)clampi(p + k, nz) * plane;
 #pragma unroll
-                for (int j = 0; j < kSqLoads; ++j)
-                    v[k][j] = (qin && in[j]) ? A[q * plane + off[j]] * scale : 0.f;
+                for (int j = 0; j < kSqLoads; ++j) v[k][j] = A[qo + off[j]];
             }
         };
-        // CT values of output planes p-1 .. p+2 (plane q completes output q-1)
+        // CT values of output planes p-1 .. p+2 (plane q completes output
+        // q-1), read only for the outputs the step writes
         float lab[kSqStep];
         auto load_labels = [&](int p) {
 #pragma unroll
-            for (int k = 0; k < kSqStep; ++k) {
-                const int q = p + k;
-                const float raw = L[(long long)clampi(q - 1, nz) * plane + col];
-                lab[k] = (q - 1 >= zs && q <= ze && col_in) ? raw : 0.f;
-            }
+            for (int k = 0; k < kSqStep; ++k)
+                lab[k] = L[(long long)clampi(p + k - 1, nz) * plane + col];
         };
         load_step(zs - 1);
         load_labels(zs - 1);
         // planes zs-1 .. ze: plane q completes output plane q-1
         for (int p = zs - 1; p <= ze; p += kSqStep) {
             __syncthreads();   // the previous step's reads of the tiles are done
+            // the loads are used only here, after the barrier: the compiler
+            // would otherwise move each under its own branch and wait for it
 #pragma unroll
             for (int k = 0; k < kSqStep; ++k)
 #pragma unroll
+                for (int j = 0; j < kSqLoads; ++j) asm volatile("" : "+v"(v[k][j]));
+#pragma unroll
+            for (int k = 0; k < kSqStep; ++k) {
+                const int q = p + k;
+                const bool qin = q >= 0 && q < nz && q <= ze;
+#pragma unroll
                 for (int j = 0; j < kSqLoads; ++j) {
                     const int e = threadIdx.x + j * kSqThreads;
-                    if (e < kSqTile) tile[k * kSqTile + e] = v[k][j];
+                    if (e < kSqTile) tile[k * kSqTile + e] = (qin && in[j]) ? v[k][j] * scale : 0.f;
                 }
+            }
             __syncthreads();
             float lab_now[kSqStep];
 #pragma unroll
