@@ -350,10 +350,10 @@ int pct_mlp_train_dev(const pct_problem* pb, float* params, const pct_train_opts
  * wrapper with cudaEvents, cu:2152-2188) ---- */
 
 /* While enabled, every fusion kernel launch (tiled or direct, and the
- * pruned kernel of pct_fuse_pruned_dev; not the AA pre-pass, the finalize
- * pass or the pruned path's item-filtered full search) is bracketed by HIP
- * events on its own stream.  Enabling clears the record.  At most 4096
- * launches are recorded. */
+ * pruned kernel of pct_fuse_pruned_dev; not the cross-term pair pre-pass,
+ * the finalize pass or the pruned path's item-filtered full search) is
+ * bracketed by HIP events on its own stream.  Enabling clears the record.
+ * At most 4096 launches are recorded. */
 int pct_kernel_timing(int enable);
 
 /* Waits for the recorded events; returns the summed kernel time and the
