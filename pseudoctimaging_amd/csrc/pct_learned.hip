@@ -711,21 +711,28 @@ __global__ __launch_bounds__(kBoxThreads) void box_xyz(const float* __restrict__
     const float* __restrict__ src = in + vi * plane * nz;
     float* __restrict__ dst = out + vi * plane * nz;
 
-    // load slots: in-plane offset of the tile element, -1 outside the
-    // volume (a zero), -2 past the tile
+    // load slots: in-plane offset of the tile element (0 where it lies
+    // outside the volume or past the tile, so that every load is
+    // unconditional and the loads of two planes stay in flight: a load under
+    // a branch makes the compiler wait for every earlier load), whether it
+    // lies in the volume (else a zero is stored), and whether it lies in the
+    // tile (else nothing is stored)
     int lofs[NLD];
+    bool lin[NLD], lst[NLD];
 #pragma unroll
     for (int i = 0; i < NLD; ++i) {
         const int e = (int)threadIdx.x + i * kBoxThreads;
         const int rr = e / C, cc = e - (e / C) * C;
         const int x = x0 - HW + rr, y = y0 - HW + cc;
-        lofs[i] = e >= R * C ? -2 : (x >= 0 && x < nx && y >= 0 && y < ny) ? x * ny + y : -1;
+        lst[i] = e < R * C;
+        lin[i] = lst[i] && x >= 0 && x < nx && y >= 0 && y < ny;
+        lofs[i] = lin[i] ? x * ny + y : 0;
     }
     float va[NLD], vb[NLD];
     auto load = [&](int p, float (&v)[NLD]) {
         const float* __restrict__ sp = src + p * plane;
 #pragma unroll
-        for (int i = 0; i < NLD; ++i) v[i] = lofs[i] >= 0 ? sp[lofs[i]] : 0.f;
+        for (int i = 0; i < NLD; ++i) v[i] = sp[lofs[i]];
     };
 
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -734,7 +741,7 @@ __global__ __launch_bounds__(kBoxThreads) void box_xyz(const float* __restrict__
     double acc[OX] = {0.0, 0.0};
     const int pa = max(0, z0 - HW), pe = z1 - 1 + HW;   // planes >= nz are zero
     load(pa, va);
-    if (pa + 1 < nz) load(pa + 1, vb);
+    load(min(pa + 1, nz - 1), vb);
     for (int pb = pa; pb <= pe; pb += KR) {
 #pragma unroll
         for (int j = 0; j < KR; ++j) {
@@ -745,11 +752,18 @@ __global__ __launch_bounds__(kBoxThreads) void box_xyz(const float* __restrict__
             if (p < nz) {        // uniform
                 __syncthreads();   // the previous plane's y sums are done with T
 #pragma unroll
+                for (int i = 0; i < NLD; ++i) asm volatile("" : "+v"(cur[i]));
+#pragma unroll
                 for (int i = 0; i < NLD; ++i) {
                     const int e = (int)threadIdx.x + i * kBoxThreads;
-                    if (lofs[i] != -2) T[(e / C) * CP + e - (e / C) * C] = cur[i];
+                    if (lst[i]) T[(e / C) * CP + e - (e / C) * C] = lin[i] ? cur[i] : 0.f;
                 }
-                if (p + 2 < nz && p + 2 <= pe) load(p + 2, cur);   // in flight for two planes
+            }
+            // plane p + 2, in flight for two planes; loaded on every path (a
+            // clamped plane where it is not needed), so that the compiler can
+            // count the loads in flight and wait for the older plane only
+            load(min(p + 2, nz - 1), cur);
+            if (p < nz) {        // uniform
                 __syncthreads();
                 if ((int)threadIdx.x < NYT) {
                     const int rr = threadIdx.x >> 4, c0 = 4 * (threadIdx.x & 15);
@@ -1330,16 +1344,12 @@ __global__ __launch_bounds__(kMergeWaves * 64) void merge_offsets_kernel(MergeAr
     };
     // the kept offsets of a dz plane in (dy, dx) order, the order in which
     // the plain kernel sweeps a slab's candidates (dy outer, dx inner), so
-    // that evaluating every candidate sums in the plain kernel's order
+    // that evaluating every candidate sums in the plain kernel's order: the
+    // rank among the kept offsets by the key (dz, dy, dx), made once per lane
     // (every lane runs the shuffles; only lanes with a kept offset use the rank)
+    const int key = has_k ? (kv / nw2) * nw2 + (kv % nw1) * nw1 + (kv / nw1) % nw1 : INT_MAX;
     int rank = 0;
-    const int kdz = kv / nw2, key = (kv % nw1) * nw1 + (kv / nw1) % nw1;
-    for (int q = 0; q < a.topk; ++q) {
-        const int o = __shfl(kv, q);
-        const int odz = o / nw2;
-        rank += o < kNoOffset &&
-                (odz < kdz || (odz == kdz && (o % nw1) * nw1 + (o / nw1) % nw1 < key));
-    }
+    for (int q = 0; q < a.topk; ++q) rank += __shfl(key, q) < key;
     int* out = a.merged + item * kMergedStride;
     if (has_k) out[1 + rank + before_k] = pack(kv, 0u);
     if (has_s && !dup)
