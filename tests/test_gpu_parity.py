@@ -5,7 +5,7 @@ import pytest
 
 import oracle
 import pseudoctimaging_amd as pct
-from parity import compare
+from parity import MAX_DHU, compare
 
 pytestmark = pytest.mark.gpu
 
@@ -70,10 +70,14 @@ def test_deterministic(gpu):
 
 # The cross-term shapes (tile configurations 15-19) weigh with
 # SSD' = TT + AA - C only where every scaled MRI value is at most 0.25 in
-# magnitude.  An item with a larger target value keeps the squared
-# differences throughout; a slab with a larger atlas value is weighed with
-# squared differences on the cross-term scale.  Each case below steers the
-# kernels into one of those paths, and every path must meet the same bar.
+# magnitude and every CT value at most 2^64.  An item with a larger target
+# value keeps the squared differences throughout; a slab with a larger atlas
+# value, or a CT value beyond 2^64 or not a number (which the pair pre-pass
+# marks with +inf), is weighed with squared differences on the cross-term
+# scale.  Each case below steers the kernels into one of those paths, and
+# every path must meet the same bar; near a CT spike of 2e19 the HU bar has no
+# meaning (a float ulp there is far above 0.5 HU), and the relative bar and
+# the NaN pattern remain.
 XT_SHAPES = [15, 16, 17, 18, 19]
 
 
@@ -88,21 +92,27 @@ def _xt_variants():
     uimg, uatlas, uct = pct.synth_uniform(40, 44, 30, 2, 12)   # values up to 10
     ulab = uct.copy()
     pct.normalize_labels(ulab)
-    return [("tissue", img, amri, lab), ("atlas-spike", img, spiked_atlas, lab),
-            ("target-spike", spiked_img, amri, lab), ("both", spiked_img, spiked_atlas, lab),
-            ("uniform", uimg, uatlas, ulab)]
+    spiked_lab = lab.copy()
+    spiked_lab[0, 14, 21, 9] = 2e19               # CT beyond 2^64: slab fallback
+    nan_lab = lab.copy()
+    nan_lab[1, 9, 30, 25] = np.nan                # CT not a number: NaN outputs around it
+    return [("tissue", img, amri, lab, MAX_DHU), ("atlas-spike", img, spiked_atlas, lab, MAX_DHU),
+            ("target-spike", spiked_img, amri, lab, MAX_DHU),
+            ("both", spiked_img, spiked_atlas, lab, MAX_DHU),
+            ("uniform", uimg, uatlas, ulab, MAX_DHU), ("ct-spike", img, amri, spiked_lab, np.inf),
+            ("ct-nan", img, amri, nan_lab, MAX_DHU)]
 
 
 @pytest.mark.parametrize("nh,sigma", [(7, 2.5), (9, 1.0), (5, 0.5)])
 def test_cross_term_range_fallbacks(gpu, nh, sigma):
     try:
-        for name, img, amri, lab in _xt_variants():
+        for name, img, amri, lab, max_dhu in _xt_variants():
             ref = oracle.fuse(img, amri, lab, nh, sigma, 1.0)
             for sid in XT_SHAPES:
                 print(f"case {name} nh={nh} sigma={sigma} shape={sid}", flush=True)
                 pct.force_tile_config(sid, 1)
                 got = pct.multi_patch_based_segmentation_gpu(img, amri, lab, nh, sigma, 1.0,
                                                              pct.CHOICE_SM)
-                compare(got, ref)
+                compare(got, ref, max_dhu=max_dhu)
     finally:
         pct.force_tile_config(-1, 0)
